@@ -1,0 +1,152 @@
+"""Throughput of the MI355X PPSieve hot path on its headline workload.
+
+Workload (single large work unit, plain kernel): Proth sign (+1), odd k in
+[1201, 9999], n in [50000, 6000000), p over the primes of [2^50, 2^50+5e10),
+exact uint64 arithmetic.  One step = one batch: the primes of the next
+p-chunk are sieved on the device (2^21 primes by default), walked over the
+whole n-range, hits filtered; the step ends when the batch's factors are on
+the host.  K steps cover a prefix of the p-range.
+
+Prints one JSON line:  value = primes/s over the timed steps (inputs
+generated on the device, nothing crosses PCIe but the factors),
+roofline = the walk kernel against the INT32 VALU peak, cpu_baseline = the
+CPU oracle (a port of the reference CPU path) on the host's cores over a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "psieve-cuda_amd", "python"))
+
+METRIC = "primes/sec sieved (k≤1e4, n≤6e6) batched vs plain; % of INT32 VALU peak"
+
+WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000, sign=1)
+
+# INT32 VALU peak of one MI355X: 256 CUs x 64 lanes x 2.4 GHz lane-ops/s
+# (published 157.3 TFLOP/s FP32 vector = that x 2 for FMA x 2 for packed FP32,
+# which integer ops do not have).
+PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32, with a
+# MUL32 costing R_MUL full-rate issue slots (measured on gfx950: v_mul_lo_u32,
+# v_mul_hi_u32 and v_mad_u64_u32 issue at a quarter of v_add_u32's rate).
+MUL32_PER_STEP = 4
+ALU32_PER_STEP = 11
+R_MUL = 4.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0, help="primes per step (0 = library default 2^21)")
+    ap.add_argument("--tune", default="", help="r,u,wg kernel shape")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(target_s):
+    """The oracle (pps_oracle.c, a C port of test_one_p + test_factor) on the
+    first primes of the workload, full n-range, one thread per host core."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pps_oracle as orc
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    pr = orc.derive(**WORKLOAD)
+    P = orc.primes(WORKLOAD["pmin"], WORKLOAD["pmin"] + 40 * 10**6)
+    t0 = time.perf_counter()
+    orc.walk(pr, P[:threads * 2], filter=True, threads=threads)
+    per = (time.perf_counter() - t0) / 2  # seconds per prime per thread, roughly
+    m = int(max(threads * 4, min(len(P), target_s / max(per, 1e-9) * threads)))
+    m = (m // threads) * threads
+    t0 = time.perf_counter()
+    orc.walk(pr, P[:m], filter=True, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": m / dt, "unit": "primes/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} primes >= 2^50 of the workload, full n-range [50000,6000000), "
+                      f"{dt:.1f} s on {threads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported (primes shard by p-range "
+              f"with no exchange; see the sharding note in the design document)", file=sys.stderr)
+        sys.exit(2)
+    import ppsd
+
+    params = ppsd.derive(ppsd.SearchParams(**WORKLOAD))
+    ctx = ppsd.Context(params, batch=a.batch)
+    if a.tune:
+        r, u, wg = (int(x) for x in a.tune.split(","))
+        ctx.set_tuning(r, u, wg)
+    p = WORKLOAD["pmin"]
+    for _ in range(a.warmup):
+        p = ctx.submit_range(p, WORKLOAD["pmax"])
+        ctx.collect()
+    ctx.reset_stats()
+    p_start = p
+    factors = 0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        p = ctx.submit_range(p, WORKLOAD["pmax"])
+        f, _, _ = ctx.collect()
+        factors += len(f)
+        print(f"step {i}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+
+    value = st.primes / dt
+    steps_per_launch = st.window_steps / max(st.walk_launches, 1)
+    t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
+    ops = MUL32_PER_STEP * R_MUL + ALU32_PER_STEP
+    achieved = steps_per_launch * ops / t_launch / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "primes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain kernel, single work unit: +1, k in [1201,9999], n in [50000,6000000), "
+                        "p prime in [2^50, 2^50+5e10) (timed prefix)",
+            "p_range_timed": [p_start, p],
+            "primes_timed": st.primes,
+            "batch_primes": ctx.batch,
+            "nstep": params.nstep,
+            "window_steps": st.window_steps,
+            "factors": factors,
+            "walk_ms": st.walk_ms,
+            "sieve_ms": st.sieve_ms,
+            "filter_ms": st.filter_ms,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_TOPS,
+            "unit": "TOP/s",
+            "frac": achieved / PEAK_TOPS,
+            "traffic": None,
+        },
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,147 @@
+/* ppsd.h -- C ABI of the MI355X (gfx950) PPSieve hot path: libppsd.so.
+ *
+ * This is the device layer that a PPSieve host calls where the reference's
+ * app.c calls its CUDA layer through pps/appcu.h.  Plain C types only; one
+ * context per device per host thread; no exit() inside the library: every
+ * entry point returns 0 or a negative code, with the text from
+ * ppsd_last_error().
+ *
+ * Semantics of a run (identical to the reference's no-input-file mode):
+ * report every (p, k, n, c) with p prime in [pmin, pmax), k == koffset
+ * (mod kstep) in [kmin, kmax], nmin <= n < nmax and p | k*2^n + c, where
+ * k*2^n + c has no prime factor 3 <= q < 32768 (pps/app.c:150-186,
+ * pps/factor_proth.c:246-253).  c = +1 (Proth), -1 (Riesel, -R) or both.
+ */
+#ifndef PPSD_H
+#define PPSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes, BOINC-compatible like the reference's (pps/app.h:64-70). */
+#define PPSD_OK 0
+#define PPSD_ERR_FAILURE (-1)                /* EXIT_FAILURE paths: main.c, checkCUDAErr appcu.cu:124-131 */
+#define PPSD_ERR_INVALID_PARAM (-178)        /* ERR_INVALID_PARAM, pps/app.h:67 */
+#define PPSD_ERR_INSUFFICIENT_RESOURCE (-198) /* ERR_INSUFFICIENT_RESOURCE, pps/app.h:69 */
+#define PPSD_ERR_NOT_IMPLEMENTED (-150)      /* ERR_NOT_IMPLEMENTED, pps/app.h:70 */
+#define PPSD_ERR_STATE (-117)                /* ERR_NEG, pps/app.h:68: call out of order */
+
+/* Search parameters.  Inputs use the reference's option meanings (main.c
+ * -p/-P, pps/app.c -k/-K/-n/-N/-M/-s/-R); 0 means "not given".
+ * ppsd_derive_params fills the outputs. */
+typedef struct ppsd_params {
+  uint64_t pmin, pmax;     /* -p, -P */
+  uint64_t kmin, kmax;     /* -k, -K */
+  uint32_t nmin, nmax;     /* -n, -N; nmax is exclusive */
+  uint32_t kstep, koffset; /* -M, -s; kstep 0 selects 2 / 1 (odd k) */
+  int32_t sign;            /* +1 Proth, -1 Riesel (-R), 0 both */
+  uint32_t nstep_ref;      /* out: the reference's nstep after its clamps */
+  uint32_t nstep;          /* out: window walked, min(nstep_ref, 32) */
+  int32_t status;          /* out: 0 or PPSD_ERR_* */
+  char msg[160];           /* out: reason for a non-zero status */
+} ppsd_params;
+
+/* One factor line "p | k*2^n+c" (pps/app.c:134). */
+typedef struct ppsd_factor {
+  uint64_t p;
+  uint64_t k;
+  uint32_t n;
+  int32_t c;
+} ppsd_factor;
+
+/* Counters of the work since ppsd_open (or the last ppsd_reset_stats). */
+typedef struct ppsd_stats {
+  uint64_t primes;        /* primes walked */
+  uint64_t prime_sum;     /* their sum mod 2^64 (main.c:1232 "sum=") */
+  uint64_t window_steps;  /* primes x windows x signs walked */
+  uint64_t raw_hits;      /* divisors found before the small-prime filter */
+  uint64_t factors;       /* factors returned */
+  uint64_t walk_launches;
+  double walk_ms;         /* walk kernels, HIP events on the context's stream */
+  double sieve_ms;        /* on-device prime supply kernels */
+  double filter_ms;       /* small-prime filter kernel */
+  uint64_t retries;       /* batches re-walked in halves after a full hit ring */
+} ppsd_stats;
+
+typedef struct ppsd_ctx ppsd_ctx;
+
+/* Parameter rules: pmin/pmax defaults (main.c:916-932), validation, nmin
+ * default, k alignment and nstep (pps/app.c:655-712), nstep clamps and range
+ * checks (pps/appcu.cu:155-209, 236-239).  Replaces the checks inside
+ * app_init + cuda_init (pps/appcu.h:31).  Returns p->status. */
+int ppsd_derive_params(ppsd_params *p);
+
+/* Number of HIP devices (0 without a GPU). */
+int ppsd_device_count(void);
+
+/* Per-worker device set-up; replaces cuda_app_init (pps/appcu.h:27,
+ * pps/appcu.cu:213-404).  p must have passed ppsd_derive_params.  batch is
+ * the number of primes per walk launch (0 = default: 32 per SIMD lane slot of
+ * the device, i.e. 2^21 on 256 CUs).  Allocates every device buffer up
+ * front; generates the sieving primes <= isqrt(pmax-1) on the device. */
+int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out);
+
+/* Primes per batch chosen by ppsd_open (the reference's cthread_count). */
+uint32_t ppsd_batch_size(const ppsd_ctx *ctx);
+
+/* Walk the n-range for n host-supplied primes (n <= batch; any order),
+ * asynchronously; replaces check_ns (pps/appcu.h:29, pps/appcu.cu:1205-1261).
+ * P is copied before return.  One submission may be outstanding. */
+int ppsd_submit_primes(ppsd_ctx *ctx, const uint64_t *P, uint32_t n);
+
+/* On-device prime supply: sieve [p0, p1) on the device and walk its primes,
+ * asynchronously.  At most one batch of primes is taken: *p_next receives
+ * where this submission stopped (p1 when the whole range fitted).
+ * No counterpart in the reference, whose primes come from the host sieve
+ * (sieve.c, main.c:750-777). */
+int ppsd_submit_range(ppsd_ctx *ctx, uint64_t p0, uint64_t p1, uint64_t *p_next);
+
+/* Wait for the outstanding submission and return its factors (filtered as
+ * test_factor does); replaces get_factors_found (pps/appcu.h:30,
+ * pps/appcu.cu:1263-1308) plus the host re-walk it fed (pps/app.c:1197-1219).
+ * Writes at most cap factors; *n_out receives the total (call again with a
+ * larger buffer when *n_out > cap: the result is kept until the next submit).
+ * primes/prime_sum (may be NULL) receive the batch's count and sum. */
+int ppsd_collect(ppsd_ctx *ctx, ppsd_factor *out, uint64_t cap, uint64_t *n_out, uint64_t *primes,
+                 uint64_t *prime_sum);
+
+/* Same wait, returning the raw divisors instead (no small-prime filter):
+ * every (p,k,n,c) with p | k*2^n+c, k odd in [kmin,kmax], nmin <= n < nmax.
+ * Either this or ppsd_collect consumes a submission. */
+int ppsd_collect_raw(ppsd_ctx *ctx, ppsd_factor *out, uint64_t cap, uint64_t *n_out, uint64_t *primes,
+                     uint64_t *prime_sum);
+
+/* Kernel shape: r primes per lane (1, 2), u windows per pre-test chunk
+ * (4, 8, 16), workgroup size (64..256, multiple of 64).  0 keeps a value. */
+int ppsd_set_tuning(ppsd_ctx *ctx, int r, int u, int wg);
+int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
+
+/* Raw-hit ring capacity for later submissions (1 .. the allocated 2^22).
+ * A batch whose raw hits overflow the ring is re-walked in halves, so the
+ * capacity only trades memory for re-walks; the result never changes. */
+int ppsd_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
+
+int ppsd_get_stats(const ppsd_ctx *ctx, ppsd_stats *st);
+int ppsd_reset_stats(ppsd_ctx *ctx);
+
+/* Frees all device state; replaces cuda_finalize (pps/appcu.h:32,
+ * pps/appcu.cu:101-122).  NULL is accepted. */
+int ppsd_close(ppsd_ctx *ctx);
+
+/* Text of the last error on this thread. */
+const char *ppsd_last_error(void);
+
+/* Arithmetic check (no counterpart in the reference): out[i] = 2^-e mod P[i]
+ * after nwin further windows of s, i.e. 2^-(e + nwin*s) mod P[i], computed by
+ * the same device code as the walk.  Requires P[i] odd in (2^32, 2^62),
+ * e >= 64, 1 <= s <= 32. */
+int ppsd_debug_state(int device, const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s,
+                     uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPSD_H */

@@ -1,0 +1,574 @@
+// ppsd_api.hip -- host half of libppsd.so: the C ABI declared in
+// include/ppsd.h.  Owns every device buffer of a context; launches the
+// kernels of ppsd_kernels.hip on one non-blocking stream per context.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ppsd.h"
+#include "ppsd_kernels.h"
+
+using namespace ppsd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                               \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error: %s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+uint64_t isqrt64(uint64_t x) {
+  uint64_t r = (uint64_t)sqrt((double)x);
+  while (r * r > x) --r;
+  while ((r + 1) * (r + 1) <= x) ++r;
+  return r;
+}
+
+int lg2(uint64_t v) {
+  int r = 0;
+  while (v >>= 1) ++r;
+  return r;
+}
+
+// odd primes <= lim (host; lim is at most 46341 here)
+std::vector<uint32_t> host_primes(uint32_t lim) {
+  std::vector<uint8_t> comp(lim + 1, 0);
+  std::vector<uint32_t> out;
+  for (uint32_t i = 3; i <= lim; i += 2) {
+    if (comp[i]) continue;
+    out.push_back(i);
+    for (uint64_t m = (uint64_t)i * i; m <= lim; m += 2 * i) comp[m] = 1;
+  }
+  return out;
+}
+
+enum class Pending { none, primes, range };
+
+}  // namespace
+
+struct ppsd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ppsd_params prm{};
+  uint32_t batch = 0;
+  int tune_r = 1, tune_u = 8, tune_wg = 256;
+
+  // device buffers
+  uint64_t *d_P = nullptr;
+  uint32_t *d_count = nullptr;             // primes in d_P (range mode)
+  unsigned long long *d_counters = nullptr;  // [0] raw hits, [1] kept, [2] prime sum
+  Hit *d_hits = nullptr;
+  Hit *d_kept = nullptr;
+  uint64_t hit_cap = 0;        // capacity in use
+  uint64_t hit_alloc = 0;      // allocated entries
+  uint32_t *d_small = nullptr;  // odd primes 11 <= q < 32768
+  uint32_t nsmall = 0;
+  uint32_t *d_sp = nullptr;     // sieving primes <= qmax
+  uint32_t nsp = 0, nsp_small = 0;
+  uint64_t qmax = 0;
+  uint32_t *d_bitmap = nullptr;
+  uint32_t *d_blockcount = nullptr;
+  uint64_t nbits_max = 0;
+  uint64_t *h_stage = nullptr;  // pinned upload staging
+
+  hipEvent_t ev[6] = {};  // walk begin/end, sieve begin/end, filter begin/end
+
+  // outstanding submission
+  Pending pending = Pending::none;
+  uint32_t n_primes_host = 0;  // prime count when known on the host
+  bool sieved = false;
+  std::vector<Hit> result;
+  bool result_raw = false;
+  uint64_t last_primes = 0, last_sum = 0;
+  bool have_result = false;
+
+  ppsd_stats st{};
+};
+
+extern "C" {
+
+int ppsd_derive_params(ppsd_params *pr) {
+  const uint64_t PMAX_MAX = (uint64_t)1 << 62;
+  pr->status = 0;
+  pr->msg[0] = 0;
+  auto fail = [&](int code, const char *m) {
+    pr->status = code;
+    snprintf(pr->msg, sizeof pr->msg, "%s", m);
+    return set_err(code, "%s", m);
+  };
+  if (pr->sign < -1 || pr->sign > 1) return fail(PPSD_ERR_INVALID_PARAM, "sign must be +1, -1 or 0");
+  if (pr->kstep == 0) {
+    pr->kstep = 2;
+    pr->koffset = 1;
+  }
+  // main.c:916-932
+  if (pr->pmin < 3) pr->pmin = 3;
+  if (pr->pmax > PMAX_MAX) pr->pmax = PMAX_MAX;
+  if (pr->pmin >= pr->pmax) {
+    if (pr->pmax == 0 && pr->pmin < PMAX_MAX - 1000000000u)
+      pr->pmax = pr->pmin + 1000000000u;
+    else
+      return fail(PPSD_ERR_FAILURE, "pmax must be greater than pmin");
+  }
+  // pps/app.c:655-712
+  if (pr->kmin == 0 || pr->kmax == 0)
+    return fail(PPSD_ERR_INVALID_PARAM, "Please specify an input file or all of kmin, kmax, and nmax");
+  if (pr->kmin > pr->kmax) return fail(PPSD_ERR_INVALID_PARAM, "kmin <= kmax is required");
+  if (pr->kmax >= pr->pmin) return fail(PPSD_ERR_INVALID_PARAM, "kmax < pmin is required");
+  if (pr->kmax - pr->kmin >= ((uint64_t)3 << 36)) return fail(PPSD_ERR_INVALID_PARAM, "kmax-kmin < 3*2^36 is required");
+  if (pr->nmin == 0) pr->nmin = (uint32_t)(2 * lg2(pr->pmin) - lg2(pr->kmax) - 1);
+  if (pr->nmax == 0) pr->nmax = pr->nmin;
+  if (pr->nmin > pr->nmax) return fail(PPSD_ERR_INVALID_PARAM, "nmin <= nmax is required");
+  const uint64_t b0 = pr->kmin / pr->kstep, b1 = pr->kmax / pr->kstep;
+  pr->kmin = b0 * pr->kstep + pr->koffset;
+  pr->kmax = b1 * pr->kstep + pr->koffset;
+  uint32_t s = 1;
+  while ((pr->kmax << s) < pr->pmin) ++s;
+  // pps/appcu.cu:155-209, 236-239
+  if (s > pr->nmax - pr->nmin + 1) s = pr->nmax - pr->nmin + 1;
+  if (((uint64_t)1 << (64 - s)) > pr->pmin)
+    return fail(PPSD_ERR_INVALID_PARAM, "pmin is not large enough (or nmax is close to nmin)");
+  if (s >= 32 && s < 48 && ((uint64_t)1 << 32) <= pr->pmin) s = 32;
+  if (lg2(pr->nmin) < 6) return fail(PPSD_ERR_INVALID_PARAM, "nmin too small (must be at least 64)");
+  if (pr->pmin <= ((uint64_t)1 << 32)) return fail(PPSD_ERR_INVALID_PARAM, "PMin is too small, <= 2^32");
+  pr->nstep_ref = s;
+  // The factor set does not depend on the window as long as
+  // kmax * 2^(s-1) < pmin; the device step is written for s <= 32.
+  pr->nstep = s > 32 ? 32 : s;
+  return 0;
+}
+
+int ppsd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char *ppsd_last_error(void) { return g_err.c_str(); }
+
+static int free_ctx(ppsd_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_P);
+  (void)hipFree(c->d_count);
+  (void)hipFree(c->d_counters);
+  (void)hipFree(c->d_hits);
+  (void)hipFree(c->d_kept);
+  (void)hipFree(c->d_small);
+  (void)hipFree(c->d_sp);
+  (void)hipFree(c->d_bitmap);
+  (void)hipFree(c->d_blockcount);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+// Sieving primes q <= qmax on the device: segmented sieve of [3, qmax] with
+// the host's odd primes <= isqrt(qmax), compacted in order into d_sp.
+static int build_sieving_primes(ppsd_ctx *c) {
+  const uint64_t qmax = c->qmax;
+  const uint32_t base_lim = (uint32_t)isqrt64(qmax);
+  std::vector<uint32_t> base = host_primes(base_lim < 3 ? 3 : base_lim);
+  // pi(x) < 1.25506 x / ln x (Rosser-Schoenfeld)
+  const uint64_t cap = qmax < 17 ? 8 : (uint64_t)(1.25506 * (double)qmax / log((double)qmax)) + 16;
+  uint32_t *d_base = nullptr;
+  HIP_TRY(hipMalloc(&c->d_sp, cap * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&d_base, (base.size() + 1) * sizeof(uint32_t)));
+  if (!base.empty())
+    HIP_TRY(hipMemcpyAsync(d_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_zero_u32(c->d_count, 1, c->stream));
+  const uint64_t hi = qmax + 1;  // sieve [3, qmax]
+  for (uint64_t lo = 3; lo < hi;) {
+    uint64_t nbits = (hi - lo + 1) / 2;
+    if (nbits > c->nbits_max) nbits = c->nbits_max;
+    SieveArgs sa{};
+    sa.lo = lo;
+    sa.nbits = nbits;
+    sa.q = d_base;
+    sa.nq_small = (uint32_t)base.size();
+    sa.nq = (uint32_t)base.size();
+    sa.bitmap = c->d_bitmap;
+    HIP_TRY(launch_sieve(sa, c->stream));
+    HIP_TRY(launch_compact32(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_sp, c->d_count, c->stream));
+    lo += 2 * nbits;
+  }
+  uint32_t n = 0;
+  HIP_TRY(hipMemcpyAsync(&n, c->d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(d_base);
+  if (n > cap) return set_err(PPSD_ERR_FAILURE, "sieving prime count %u exceeds bound %llu", n, (unsigned long long)cap);
+  c->nsp = n;
+  // primes below 2^16 are marked per LDS segment, the rest with atomics
+  uint32_t small = 0;
+  for (uint32_t q : host_primes(65535))
+    if (q <= qmax) ++small;
+  c->nsp_small = std::min(small, n);
+  return 0;
+}
+
+int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) {
+  if (!p || !out) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  *out = nullptr;
+  if (p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) || p->nmin < 64)
+    return set_err(PPSD_ERR_INVALID_PARAM, "parameters have not passed ppsd_derive_params");
+  int ndev = ppsd_device_count();
+  if (device < 0 || device >= ndev) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d not available (%d devices)", device, ndev);
+  ppsd_ctx *c = new ppsd_ctx;
+  c->device = device;
+  c->prm = *p;
+  c->batch = batch ? batch : (1u << 21);
+  c->hit_cap = c->hit_alloc = 1u << 22;
+  c->nbits_max = (uint64_t)1 << 27;
+  int rc = 0;
+  auto bail = [&](int code) {
+    std::string keep = g_err;
+    free_ctx(c);
+    g_err = keep;
+    return code;
+  };
+#define OPEN_TRY(expr)                                                                                \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      return bail(set_err(e_ == hipErrorOutOfMemory ? PPSD_ERR_INSUFFICIENT_RESOURCE : PPSD_ERR_FAILURE, \
+                          "HIP error: %s: %s", #expr, hipGetErrorString(e_)));                        \
+  } while (0)
+  OPEN_TRY(hipSetDevice(device));
+  OPEN_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto &e : c->ev) OPEN_TRY(hipEventCreate(&e));
+  OPEN_TRY(hipMalloc(&c->d_P, (size_t)c->batch * sizeof(uint64_t)));
+  OPEN_TRY(hipMalloc(&c->d_count, sizeof(uint32_t)));
+  OPEN_TRY(hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)));
+  OPEN_TRY(hipMalloc(&c->d_hits, c->hit_cap * sizeof(Hit)));
+  OPEN_TRY(hipMalloc(&c->d_kept, c->hit_cap * sizeof(Hit)));
+  OPEN_TRY(hipMalloc(&c->d_bitmap, (c->nbits_max / 32 + 1) * sizeof(uint32_t)));
+  OPEN_TRY(hipMalloc(&c->d_blockcount, (c->nbits_max / 32 / kScanBlock + 2) * sizeof(uint32_t)));
+  OPEN_TRY(hipHostMalloc((void **)&c->h_stage, (size_t)c->batch * sizeof(uint64_t), hipHostMallocDefault));
+  {
+    std::vector<uint32_t> sm;
+    for (uint32_t q : host_primes(32767))
+      if (q >= 11) sm.push_back(q);
+    c->nsmall = (uint32_t)sm.size();
+    OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(uint32_t)));
+    OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  c->qmax = isqrt64(p->pmax - 1);
+  rc = build_sieving_primes(c);
+  if (rc) return bail(rc);
+#undef OPEN_TRY
+  *out = c;
+  return 0;
+}
+
+uint32_t ppsd_batch_size(const ppsd_ctx *c) { return c ? c->batch : 0; }
+
+int ppsd_set_tuning(ppsd_ctx *c, int r, int u, int wg) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (r && r != 1 && r != 2) return set_err(PPSD_ERR_INVALID_PARAM, "r must be 1 or 2");
+  if (u && u != 4 && u != 8 && u != 16) return set_err(PPSD_ERR_INVALID_PARAM, "u must be 4, 8 or 16");
+  if (wg && (wg < 64 || wg > 256 || wg % 64)) return set_err(PPSD_ERR_INVALID_PARAM, "wg must be 64..256, multiple of 64");
+  if (r) c->tune_r = r;
+  if (u) c->tune_u = u;
+  if (wg) c->tune_wg = wg;
+  return 0;
+}
+
+int ppsd_get_tuning(const ppsd_ctx *c, int *r, int *u, int *wg) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (r) *r = c->tune_r;
+  if (u) *u = c->tune_u;
+  if (wg) *wg = c->tune_wg;
+  return 0;
+}
+
+int ppsd_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (cap < 1 || cap > c->hit_alloc) return set_err(PPSD_ERR_INVALID_PARAM, "capacity must be 1..%llu", (unsigned long long)c->hit_alloc);
+  if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "submission outstanding");
+  c->hit_cap = cap;
+  return 0;
+}
+
+int ppsd_get_stats(const ppsd_ctx *c, ppsd_stats *st) {
+  if (!c || !st) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  *st = c->st;
+  return 0;
+}
+
+int ppsd_reset_stats(ppsd_ctx *c) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  c->st = ppsd_stats{};
+  return 0;
+}
+
+static uint32_t windows(const ppsd_params &p) {
+  return p.nmax > p.nmin ? (p.nmax - p.nmin + p.nstep - 1) / p.nstep : 0;
+}
+
+// Walk primes d_P[first .. first+n) (or the device count when count_d is set)
+// into the raw ring, then filter; counters must be zero.
+static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t *count_d, uint32_t grid_primes) {
+  WalkArgs a{};
+  a.P = c->d_P + first;
+  a.count_d = count_d;
+  a.count = n;
+  a.nmin = c->prm.nmin;
+  a.nmax = c->prm.nmax;
+  a.s = c->prm.nstep;
+  a.nwin = windows(c->prm);
+  a.kmin = c->prm.kmin;
+  a.kmax = c->prm.kmax;
+  a.hits = c->d_hits;
+  a.hit_count = c->d_counters;
+  a.hit_cap = c->hit_cap;
+  const uint32_t per_block = (uint32_t)(c->tune_wg * c->tune_r);
+  const uint32_t grid = (grid_primes + per_block - 1) / per_block;
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  if (grid) HIP_TRY(launch_walk(a, c->prm.sign, c->tune_r, c->tune_u, grid, (uint32_t)c->tune_wg, c->stream));
+  HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  FilterArgs f{};
+  f.hits = c->d_hits;
+  f.hit_count = c->d_counters;
+  f.hit_cap = c->hit_cap;
+  f.kmin = c->prm.kmin;
+  f.kmax = c->prm.kmax;
+  f.nmin = c->prm.nmin;
+  f.nmax = c->prm.nmax;
+  f.kstep = c->prm.kstep;
+  f.koffset = c->prm.koffset;
+  f.small_primes = c->d_small;
+  f.nsmall = c->nsmall;
+  f.kept = c->d_kept;
+  f.kept_count = c->d_counters + 1;
+  f.kept_cap = c->hit_cap;
+  HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+  HIP_TRY(launch_filter(f, 1024, c->stream));
+  HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+  c->st.walk_launches++;
+  return 0;
+}
+
+int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
+  if (!c || (!P && n)) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "previous submission not collected");
+  if (n > c->batch) return set_err(PPSD_ERR_INVALID_PARAM, "%u primes exceed the batch size %u", n, c->batch);
+  const uint64_t top = (uint64_t)1 << 62;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t p = P[i];
+    if (!(p & 1) || p < c->prm.pmin || p >= top)
+      return set_err(PPSD_ERR_INVALID_PARAM, "P[%u]=%llu is not odd in [pmin, 2^62)", i, (unsigned long long)p);
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  memcpy(c->h_stage, P, (size_t)n * sizeof(uint64_t));
+  HIP_TRY(hipMemcpyAsync(c->d_P, c->h_stage, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_zero_u64(c->d_counters, 4, c->stream));
+  int rc = enqueue_walk(c, 0, n, nullptr, n);
+  if (rc) return rc;
+  c->pending = Pending::primes;
+  c->n_primes_host = n;
+  c->sieved = false;
+  c->have_result = false;
+  uint64_t sum = 0;
+  for (uint32_t i = 0; i < n; ++i) sum += P[i];
+  c->last_primes = n;
+  c->last_sum = sum;
+  return 0;
+}
+
+int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "previous submission not collected");
+  if (p0 < c->prm.pmin || p1 > c->prm.pmax || p0 > p1)
+    return set_err(PPSD_ERR_INVALID_PARAM, "range [%llu,%llu) outside [pmin,pmax)", (unsigned long long)p0,
+                   (unsigned long long)p1);
+  HIP_TRY(hipSetDevice(c->device));
+  const uint64_t lo = p0 | 1;
+  uint64_t nbits = p1 > lo ? (p1 - lo + 1) / 2 : 0;
+  // keep the expected prime count (2 per ln x odd candidates) 10% under the
+  // batch; segment-aligned so chunks tile the range
+  const double dens = 2.0 / log((double)lo);
+  uint64_t fit = (uint64_t)(0.9 * c->batch / dens);
+  fit = std::min<uint64_t>(fit, c->nbits_max);
+  fit = fit / kSegBits * kSegBits;
+  if (fit == 0) fit = kSegBits;
+  if (nbits > fit) nbits = fit;
+  const uint64_t stop = nbits ? std::min<uint64_t>(lo + 2 * nbits, p1) : p1;
+  HIP_TRY(launch_zero_u64(c->d_counters, 4, c->stream));
+  HIP_TRY(launch_zero_u32(c->d_count, 1, c->stream));
+  if (nbits) {
+    SieveArgs sa{};
+    sa.lo = lo;
+    sa.nbits = nbits;
+    sa.q = c->d_sp;
+    sa.nq_small = c->nsp_small;
+    sa.nq = c->nsp;
+    sa.bitmap = c->d_bitmap;
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(launch_sieve(sa, c->stream));
+    HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->d_count, c->d_counters + 2,
+                             c->stream));
+    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  }
+  int rc = enqueue_walk(c, 0, 0, c->d_count, c->batch);
+  if (rc) return rc;
+  c->pending = Pending::range;
+  c->sieved = nbits != 0;
+  c->have_result = false;
+  if (p_next) *p_next = stop;
+  return 0;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+// Wait for the submission; gather kept (and raw) hits, re-walking in halves
+// when the raw ring overflowed.
+static int finish(ppsd_ctx *c, bool raw) {
+  if (c->pending == Pending::none) {
+    if (c->have_result && c->result_raw == raw) return 0;
+    return set_err(PPSD_ERR_STATE, "nothing submitted");
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned long long cnt[4];
+  uint32_t nprimes = c->n_primes_host;
+  HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+  if (c->pending == Pending::range) HIP_TRY(hipMemcpyAsync(&nprimes, c->d_count, sizeof nprimes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
+  c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
+  if (c->sieved) c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);
+  if (c->pending == Pending::range) {
+    c->last_primes = nprimes;
+    c->last_sum = cnt[2];
+    if (nprimes > c->batch) {
+      c->pending = Pending::none;
+      return set_err(PPSD_ERR_FAILURE, "device prime count %u exceeded the batch %u", nprimes, c->batch);
+    }
+  }
+  c->result.clear();
+  c->result_raw = raw;
+  uint64_t raw_total = 0, kept_total = 0;
+  // pieces to (re)walk: (first, n)
+  std::vector<std::pair<uint32_t, uint32_t>> todo;
+  if (cnt[0] <= c->hit_cap) {
+    const uint64_t take = raw ? cnt[0] : cnt[1];
+    const size_t old = c->result.size();
+    c->result.resize(old + take);
+    if (take)
+      HIP_TRY(hipMemcpy(c->result.data() + old, raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
+    raw_total += cnt[0];
+    kept_total += cnt[1];
+  } else {
+    c->st.retries++;
+    todo.push_back({0, nprimes / 2});
+    todo.push_back({nprimes / 2, nprimes - nprimes / 2});
+  }
+  while (!todo.empty()) {
+    auto [first, n] = todo.back();
+    todo.pop_back();
+    if (n == 0) continue;
+    HIP_TRY(launch_zero_u64(c->d_counters, 2, c->stream));
+    int rc = enqueue_walk(c, first, n, nullptr, n);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
+    c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
+    if (cnt[0] > c->hit_cap) {
+      if (n == 1) return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "one prime has more than %llu hits", (unsigned long long)c->hit_cap);
+      c->st.retries++;
+      todo.push_back({first, n / 2});
+      todo.push_back({first + n / 2, n - n / 2});
+      continue;
+    }
+    const uint64_t take = raw ? cnt[0] : cnt[1];
+    const size_t old = c->result.size();
+    c->result.resize(old + take);
+    if (take)
+      HIP_TRY(hipMemcpy(c->result.data() + old, raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
+    raw_total += cnt[0];
+    kept_total += cnt[1];
+  }
+  const uint64_t signs = c->prm.sign == 0 ? 2 : 1;
+  c->st.primes += nprimes;
+  c->st.prime_sum += c->last_sum;
+  c->st.window_steps += (uint64_t)nprimes * windows(c->prm) * signs;
+  c->st.raw_hits += raw_total;
+  c->st.factors += kept_total;
+  c->pending = Pending::none;
+  c->have_result = true;
+  return 0;
+}
+
+static int collect(ppsd_ctx *c, bool raw, ppsd_factor *out, uint64_t cap, uint64_t *n_out, uint64_t *primes,
+                   uint64_t *psum) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  int rc = finish(c, raw);
+  if (rc) return rc;
+  const uint64_t n = c->result.size();
+  if (n_out) *n_out = n;
+  if (primes) *primes = c->last_primes;
+  if (psum) *psum = c->last_sum;
+  static_assert(sizeof(Hit) == sizeof(ppsd_factor), "layout");
+  if (out && cap) memcpy(out, c->result.data(), (size_t)std::min<uint64_t>(n, cap) * sizeof(Hit));
+  return 0;
+}
+
+int ppsd_collect(ppsd_ctx *c, ppsd_factor *out, uint64_t cap, uint64_t *n_out, uint64_t *primes, uint64_t *psum) {
+  return collect(c, false, out, cap, n_out, primes, psum);
+}
+
+int ppsd_collect_raw(ppsd_ctx *c, ppsd_factor *out, uint64_t cap, uint64_t *n_out, uint64_t *primes,
+                     uint64_t *psum) {
+  return collect(c, true, out, cap, n_out, primes, psum);
+}
+
+int ppsd_close(ppsd_ctx *c) { return free_ctx(c); }
+
+int ppsd_debug_state(int device, const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s,
+                     uint64_t *out) {
+  if ((!P || !out) && n) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  if (e < 64 || s < 1 || s > 32) return set_err(PPSD_ERR_INVALID_PARAM, "need e >= 64 and 1 <= s <= 32");
+  for (uint32_t i = 0; i < n; ++i)
+    if (!(P[i] & 1) || P[i] <= ((uint64_t)1 << 32) || P[i] >= ((uint64_t)1 << 62))
+      return set_err(PPSD_ERR_INVALID_PARAM, "P[%u] not odd in (2^32, 2^62)", i);
+  if (device < 0 || device >= ppsd_device_count()) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d not available", device);
+  if (n == 0) return 0;
+  HIP_TRY(hipSetDevice(device));
+  uint64_t *d = nullptr;
+  HIP_TRY(hipMalloc(&d, 2 * (size_t)n * sizeof(uint64_t)));
+  hipError_t err = hipMemcpy(d, P, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = launch_state(d, n, e, nwin, s, d + n, nullptr);
+  if (err == hipSuccess) err = hipMemcpy(out, d + n, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error: %s", hipGetErrorString(err));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// ppsd_arith.h -- exact 64-bit modular arithmetic for the PPSieve n-walk on
+// gfx950 (wave64).  Integer only; no floating point reaches a residue.
+//
+// Notation: p odd, 2^32 < p < 2^62.  pinv = p^-1 mod 2^64 (positive inverse).
+// REDC is used in its "difference" form: for T < p*2^64 and m = lo64(T)*pinv,
+// lo64(m*p) == lo64(T), so T*2^-64 == hi64(T) - hi64(m*p) (mod p) exactly,
+// and the difference lies in (-p, p): one conditional +p makes it canonical.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppsd {
+
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
+
+// p^-1 mod 2^64 by Newton's iteration x <- x(2 - p x), starting from x = p,
+// which is correct to 3 bits because p^2 == 1 (mod 8): 3, 6, 12, 24, 48 bits
+// in 32-bit arithmetic, then one 64-bit step.
+__device__ __forceinline__ uint64_t inv64(uint64_t p) {
+  const uint32_t p32 = lo32(p);
+  uint32_t x = p32;
+  x *= 2u - p32 * x;
+  x *= 2u - p32 * x;
+  x *= 2u - p32 * x;
+  x *= 2u - p32 * x;
+  uint64_t y = x;
+  y *= 2u - p * y;
+  return y;
+}
+
+// a*b*2^-64 mod p for a, b < p (canonical result).
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t pinv) {
+  const uint64_t lo = a * b;
+  const uint64_t hi = __umul64hi(a, b);
+  const uint64_t m = lo * pinv;
+  const uint64_t mh = __umul64hi(m, p);
+  uint64_t t;
+  const bool neg = __builtin_sub_overflow(hi, mh, &t);
+  return neg ? t + p : t;
+}
+
+// x/2 mod p for canonical x.
+__device__ __forceinline__ uint64_t half_mod(uint64_t x, uint64_t p) { return (x & 1) ? (x + p) >> 1 : x >> 1; }
+
+// 2^-e mod p (canonical) for e >= 32 and p > 2^32.  The Montgomery form of
+// 2^-t is 2^(64-t); starting from t = the top six bits of e (32 <= t <= 63)
+// it is already reduced, and the remaining bits of e are consumed left to
+// right: square, then halve where the bit is set.  A final REDC leaves
+// Montgomery form.
+__device__ __forceinline__ uint64_t inv2pow(uint64_t p, uint64_t pinv, uint32_t e) {
+  const int L = 32 - __clz((int)e);  // bit length of e, >= 6
+  const uint32_t t = e >> (L - 6);
+  uint64_t x = (uint64_t)1 << (64 - t);
+  for (int b = L - 7; b >= 0; --b) {
+    x = mont_mul(x, x, p, pinv);
+    if ((e >> b) & 1u) x = half_mod(x, p);
+  }
+  // REDC(x) = x * 2^-64: T = x, hi(T) = 0
+  const uint64_t m = x * pinv;
+  const uint64_t mh = __umul64hi(m, p);
+  return mh ? p - mh : 0;
+}
+
+// One window: K <- K * 2^-s (mod p) for 1 <= s <= 32.
+//   j = K * p^-1 mod 2^s (only K's low word matters), so K - j*p == 0 mod 2^s
+//   and (K - j*p) / 2^s = (K >> s) - ((j*p) >> s), the second term computed as
+//   (m*p) >> 32 = m*p_hi + mulhi(m, p_lo) with m = j << (32-s).  Folding the
+//   shift into the inverse, m = K_lo * (pinv32 << (32-s)) mod 2^32.
+// The state is kept redundant: K' = (K >> s) + (p - jp) lies in
+// [1, p + p/(2^s - 1)) whenever K does, so no per-step reduction is needed;
+// K >= p (rare: needs K >> s >= jp) is caught by the pre-test and reduced
+// only where a candidate is examined.
+template <bool S32>
+__device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t pinv_sh, uint32_t s) {
+  const uint32_t m = lo32(K) * pinv_sh;
+  const uint32_t h = __umulhi(m, lo32(p));
+  const uint64_t jp = (uint64_t)m * hi32(p) + h;
+  const uint64_t ks = S32 ? (uint64_t)hi32(K) : (K >> s);
+  // (ks - jp) wraps when negative; adding p brings it back into range
+  uint64_t d;
+  __builtin_sub_overflow(ks, jp, &d);
+  return d + p;
+}
+
+// Find-first-set-bit of the low word, as v_ffbl_b32: 0xffffffff when the word
+// is zero.  Written as asm so a zero word is defined behaviour (no hit is
+// possible then, because the trailing-zero count is >= 32 >= s; the shifted
+// value only feeds a pre-test that the exact re-check repeats).
+__device__ __forceinline__ uint32_t ffbl(uint32_t x) {
+  uint32_t r;
+  asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// Necessary condition for a hit in the window whose reduced state is K: the
+// odd part of K must not exceed kmax.  (A state not yet reduced, K >= p, is
+// caught separately by the caller.)
+__device__ __forceinline__ bool pretest(uint64_t K, uint64_t kmax) {
+  const uint32_t i = ffbl(lo32(K));
+  return (K >> (i & 63u)) <= kmax;
+}
+
+}  // namespace ppsd

@@ -1,0 +1,75 @@
+// ppsd_kernels.h -- kernel argument blocks and launch helpers shared by the
+// kernels (ppsd_kernels.hip) and the C-ABI host layer (ppsd_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppsd {
+
+// One raw hit: p | k*2^n + c, with k odd in [kmin,kmax] and nmin <= n < nmax.
+// Same field order and widths as ppsd_factor in include/ppsd.h.
+struct Hit {
+  uint64_t p;
+  uint64_t k;
+  uint32_t n;
+  int32_t c;
+};
+
+struct WalkArgs {
+  const uint64_t *P;        // primes, any order
+  const uint32_t *count_d;  // prime count on the device (range mode) or null
+  uint32_t count;           // prime count when count_d is null
+  uint32_t nmin, nmax;      // nmax exclusive
+  uint32_t s;               // window, 1..32
+  uint32_t nwin;            // windows to walk: ceil((nmax - nmin) / s)
+  uint64_t kmin, kmax;
+  Hit *hits;                         // raw-hit ring
+  unsigned long long *hit_count;     // slots requested (may exceed cap)
+  uint64_t hit_cap;
+};
+
+struct FilterArgs {
+  const Hit *hits;
+  const unsigned long long *hit_count;
+  uint64_t hit_cap;
+  uint64_t kmin, kmax;  // unused by the rule itself, kept for symmetry
+  uint32_t nmin, nmax;
+  uint32_t kstep, koffset;
+  const uint32_t *small_primes;  // odd primes 11 <= q < 32768
+  uint32_t nsmall;
+  Hit *kept;
+  unsigned long long *kept_count;
+  uint64_t kept_cap;
+};
+
+struct SieveArgs {
+  uint64_t lo;       // odd; bit b <-> lo + 2b
+  uint64_t nbits;    // candidates in this chunk
+  const uint32_t *q; // sieving primes, ascending
+  uint32_t nq_small; // q[0..nq_small) are marked per segment in LDS
+  uint32_t nq;       // q[nq_small..nq) are marked with global atomics
+  uint32_t *bitmap;  // ceil(nbits/32) words; set bit = composite
+};
+
+// Segment geometry of the LDS sieve: 2^18 candidates (2^19 integers) per
+// workgroup, 32 KiB of LDS.
+constexpr uint32_t kSegBits = 1u << 18;
+constexpr uint32_t kSegWords = kSegBits / 32;
+constexpr uint32_t kScanBlock = 256;  // words per compaction block
+
+hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block,
+                       hipStream_t st);
+hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st);
+hipError_t launch_sieve(const SieveArgs &a, hipStream_t st);
+// Compaction: primes of the chunk to out (ascending), appended at *count_d;
+// *sum_d accumulates their sum mod 2^64.
+hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
+                            uint64_t *out, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st);
+hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
+                            uint32_t *out, uint32_t *count_d, hipStream_t st);
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);
+hipError_t launch_state(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s, uint64_t *out,
+                        hipStream_t st);
+
+}  // namespace ppsd

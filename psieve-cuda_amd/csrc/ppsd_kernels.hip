@@ -1,0 +1,482 @@
+// ppsd_kernels.hip -- gfx950 kernels of the PPSieve hot path.
+//
+//   walk_kernel      the n-walk (replaces d_start_ns + d_check_more_ns*,
+//                    pps/appcu.cu:1117-1190): one lane per prime, exact hits
+//                    compacted in-wave into a device ring
+//   filter_kernel    test_factor + try_all_factors on every raw hit
+//                    (pps/app.c:150-186, pps/factor_proth.c:220-253)
+//   sieve_* / compact_*  segmented Eratosthenes on the device
+//                    (replaces sieve.c + main.c:750-777)
+//   state_kernel     2^-e mod p then nwin windows; arithmetic check only
+#include "ppsd_arith.h"
+#include "ppsd_kernels.h"
+
+namespace ppsd {
+
+// ---------------------------------------------------------------------------
+// hit emission: ballot + prefix inside the wave, one atomic per wave
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void emit(bool hit, uint64_t p, uint64_t k, uint32_t n, int32_t c,
+                                     const WalkArgs &a) {
+  const uint64_t mask = __ballot(hit);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if ((int)lane == leader) base = atomicAdd(a.hit_count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (hit) {
+    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    const uint64_t idx = base + (uint64_t)__popcll(below);
+    if (idx < a.hit_cap) {
+      Hit h;
+      h.p = p;
+      h.k = k;
+      h.n = n;
+      h.c = c;
+      a.hits[idx] = h;
+    }
+  }
+}
+
+// Exact window test (pps/app.c:1093-1109 semantics) on a state reduced to
+// [0, p): K = k*2^i with i < s, kmin <= k <= kmax, n+i inside [nmin, nmax).
+__device__ __forceinline__ bool exact(uint64_t K, uint32_t n, const WalkArgs &a, uint64_t &k, uint32_t &nn) {
+  const uint32_t lo = lo32(K);
+  if (lo == 0) return false;
+  const uint32_t i = (uint32_t)__builtin_ctz(lo);
+  if (i >= a.s) return false;
+  k = K >> i;
+  nn = n + i;
+  return k <= a.kmax && k >= a.kmin && nn < a.nmax;
+}
+
+// Pre-test of one window: a state not yet reduced (K >= p), or an odd part
+// <= kmax for the walked sign (both signs: also for p - K).
+template <int SIGN>
+__device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
+  const bool unreduced = K >= p;
+  if (SIGN == 0) return (int)unreduced | (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
+  return (int)unreduced | (int)pretest(K, kmax);
+}
+
+// Re-walk `cnt` windows from state K (window start n) with exact tests and
+// emission.  Reached only when a pre-test fired somewhere in the chunk.
+template <int SIGN, int R, bool S32>
+__device__ __noinline__ void rescan(const uint64_t (&p)[R], const uint32_t (&pinv_sh)[R], const bool (&valid)[R],
+                                    const uint64_t (&K0)[R], uint32_t n, uint32_t cnt, const WalkArgs &a) {
+  uint64_t K[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) K[r] = K0[r];
+  for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t Kc = K[r] >= p[r] ? K[r] - p[r] : K[r];
+      uint64_t k = 0;
+      uint32_t nn = 0;
+      if (SIGN == 0) {
+        bool h = valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
+        emit(h, p[r], k, nn, -1, a);
+        h = valid[r] && exact(p[r] - Kc, n, a, k, nn);
+        emit(h, p[r], k, nn, +1, a);
+      } else {
+        const bool h = valid[r] && exact(Kc, n, a, k, nn);
+        emit(h, p[r], k, nn, SIGN, a);
+      }
+      K[r] = step<S32>(K[r], p[r], pinv_sh[r], a.s);
+    }
+  }
+}
+
+// The walk.  Lane g owns primes g*R .. g*R+R-1.  K starts at -c*2^-nmin mod p
+// (for both signs: 2^-nmin, whose negation p-K is the +1 form) and advances
+// one window of s exponents per step.  Steps run in chunks of U; a chunk only
+// leaves the straight-line path when one of its pre-tests fired.
+template <int SIGN, int R, int U, bool S32>
+__global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
+  const uint32_t count = a.count_d ? *a.count_d : a.count;
+  const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * R;
+  if (first >= count) return;
+  uint64_t p[R], K[R];
+  uint32_t pinv_sh[R];
+  bool valid[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    valid[r] = first + r < count;
+    p[r] = a.P[valid[r] ? first + r : count - 1];
+    const uint64_t pinv = inv64(p[r]);
+    pinv_sh[r] = lo32(pinv) << (32u - a.s);
+    const uint64_t x = inv2pow(p[r], pinv, a.nmin);
+    K[r] = (SIGN == 1) ? p[r] - x : x;
+  }
+  const uint32_t s = a.s;
+  uint32_t n = a.nmin;
+  uint32_t w = 0;
+  for (; w + U <= a.nwin; w += U) {
+    uint64_t Ks[R];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) Ks[r] = K[r];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        any |= pre<SIGN>(K[r], p[r], a.kmax);
+        K[r] = step<S32>(K[r], p[r], pinv_sh[r], s);
+      }
+    }
+    if (any) rescan<SIGN, R, S32>(p, pinv_sh, valid, Ks, n, U, a);
+    n += U * s;
+  }
+  for (; w < a.nwin; ++w, n += s) {
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) any |= pre<SIGN>(K[r], p[r], a.kmax);
+    if (any) rescan<SIGN, R, S32>(p, pinv_sh, valid, K, n, 1, a);
+#pragma unroll
+    for (int r = 0; r < R; ++r) K[r] = step<S32>(K[r], p[r], pinv_sh[r], s);
+  }
+}
+
+template <int SIGN, int R, bool S32>
+static hipError_t launch_walk_u(const WalkArgs &a, int u, uint32_t grid, uint32_t block, hipStream_t st) {
+  switch (u) {
+    case 4: hipLaunchKernelGGL((walk_kernel<SIGN, R, 4, S32>), dim3(grid), dim3(block), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((walk_kernel<SIGN, R, 8, S32>), dim3(grid), dim3(block), 0, st, a); break;
+    case 16: hipLaunchKernelGGL((walk_kernel<SIGN, R, 16, S32>), dim3(grid), dim3(block), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int SIGN, bool S32>
+static hipError_t launch_walk_r(const WalkArgs &a, int r, int u, uint32_t grid, uint32_t block, hipStream_t st) {
+  switch (r) {
+    case 1: return launch_walk_u<SIGN, 1, S32>(a, u, grid, block, st);
+    case 2: return launch_walk_u<SIGN, 2, S32>(a, u, grid, block, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <bool S32>
+static hipError_t launch_walk_s(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block,
+                                hipStream_t st) {
+  switch (sign) {
+    case 1: return launch_walk_r<1, S32>(a, r, u, grid, block, st);
+    case -1: return launch_walk_r<-1, S32>(a, r, u, grid, block, st);
+    case 0: return launch_walk_r<0, S32>(a, r, u, grid, block, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block, hipStream_t st) {
+  if (a.s < 1 || a.s > 32) return hipErrorInvalidValue;
+  return a.s == 32 ? launch_walk_s<true>(a, sign, r, u, grid, block, st)
+                   : launch_walk_s<false>(a, sign, r, u, grid, block, st);
+}
+
+// ---------------------------------------------------------------------------
+// filter: test_factor (pps/app.c:159-186) on every raw hit, one wave per hit
+// ---------------------------------------------------------------------------
+
+// x*y mod q for x, y < q < 2^15: the float quotient estimate is off by at most
+// one (relative error ~2^-22 on a quotient below 2^15), so two corrections
+// make it exact.
+__device__ __forceinline__ uint32_t mulmod_q(uint32_t x, uint32_t y, uint32_t q, float rq) {
+  const uint32_t prod = x * y;
+  const uint32_t e = (uint32_t)((float)prod * rq);
+  int32_t r = (int32_t)(prod - e * q);
+  r += (r < 0) ? (int32_t)q : 0;
+  r -= (r >= (int32_t)q) ? (int32_t)q : 0;
+  return (uint32_t)r;
+}
+
+// q | k*2^n + c ?
+__device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, uint32_t q) {
+  const float rq = __frcp_rn((float)q);
+  uint32_t t = 1;
+  for (int b = 31 - __clz((int)n); b >= 0; --b) {
+    t = mulmod_q(t, t, q, rq);
+    if ((n >> b) & 1u) {
+      t <<= 1;
+      t -= (t >= q) ? q : 0;
+    }
+  }
+  const uint32_t km = (uint32_t)(k % q);
+  const uint32_t v = mulmod_q(km, t, q, rq);
+  return (c > 0) ? (v + 1 == q) : (v == 1);
+}
+
+// The residue pre-tests of test_factor in wrapping 64-bit arithmetic with their
+// k-size guards; mod31 receives (k << (n % 10)) + c when that branch is
+// reached and stays 1 otherwise, as in the reference.
+__device__ __forceinline__ bool pre_rules(uint64_t k, uint32_t n, int32_t c, uint64_t &mod31) {
+  const uint64_t cc = (uint64_t)(int64_t)c;
+  const uint32_t khigh = (uint32_t)(k >> 32);
+  const uint32_t r15 = (uint32_t)(((k << (n & 3)) + cc) % 15);
+  // 1 where r15 is divisible by neither 3 nor 5 (pps/app.c:152)
+  const bool ok15 = (0x6996u >> r15) & 1u;
+  mod31 = 1;
+  bool keep = ok15 && (uint32_t)(((k << (n % 3)) + cc) % 7) != 0;
+  if (keep && khigh < (1u << 24)) {
+    keep = (uint32_t)(((k << (n & 7)) + cc) % 17) != 0;
+    if (keep && khigh < (1u << 22)) {
+      mod31 = (k << (n % 10)) + cc;
+      keep = (uint32_t)(mod31 % 11) != 0;
+      if (keep && khigh < (1u << 21)) {
+        keep = (uint32_t)(((k << (n % 11)) + cc) % 23) != 0;
+        if (keep && khigh < (1u << 20)) {
+          keep = (uint32_t)(((k << (n % 12)) + cc) % 13) != 0;
+          if (keep && khigh < (1u << 14)) keep = (uint32_t)(((k << (n % 18)) + cc) % 19) != 0;
+        }
+      }
+    }
+  }
+  return keep;
+}
+
+__global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
+  const unsigned long long raw = *a.hit_count;
+  const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = wave; h < nraw; h += nwaves) {
+    const Hit x = a.hits[h];
+    const uint64_t b = x.k / a.kstep;
+    bool keep = x.k == (uint64_t)a.kstep * b + a.koffset && x.n >= a.nmin && x.n < a.nmax;
+    uint64_t mod31 = 1;
+    if (keep) keep = pre_rules(x.k, x.n, x.c, mod31) && (uint32_t)(mod31 % 31) != 0;
+    if (keep) {
+      bool d = false;
+      for (uint32_t j = lane; j < a.nsmall; j += 64) d |= divides(x.k, x.n, x.c, a.small_primes[j]);
+      keep = __ballot(d) == 0;
+    }
+    if (keep && lane == 0) {
+      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
+      if (idx < a.kept_cap) a.kept[idx] = x;
+    }
+  }
+}
+
+hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// segmented sieve of Eratosthenes over odd candidates
+// ---------------------------------------------------------------------------
+
+// First odd multiple of q that is >= max(q^2, from); from is odd.
+__device__ __forceinline__ uint64_t first_multiple(uint64_t from, uint32_t q) {
+  const uint64_t qq = (uint64_t)q * q;
+  if (qq >= from) return qq;
+  const uint32_t r = (uint32_t)(from % q);
+  uint64_t m = from + (r ? q - r : 0);
+  if ((m & 1) == 0) m += q;
+  return m;
+}
+
+// One workgroup per segment of kSegBits candidates.  Primes below 1024 are
+// marked cooperatively by the whole workgroup (each has many multiples in a
+// segment); the rest of q[0..nq_small) one prime per thread.  Every bitmap
+// word of the segment is written, tail bits past nbits set.
+__global__ __launch_bounds__(256) void sieve_small_kernel(SieveArgs a) {
+  __shared__ uint32_t seg[kSegWords];
+  const uint64_t b0 = (uint64_t)blockIdx.x * kSegBits;
+  const uint64_t rem = a.nbits - b0;
+  const uint32_t nb = rem < kSegBits ? (uint32_t)rem : kSegBits;
+  const uint64_t seg_lo = a.lo + 2 * b0;
+  for (uint32_t w = threadIdx.x; w < kSegWords; w += blockDim.x) seg[w] = 0;
+  __syncthreads();
+  uint32_t i = 0;
+  for (; i < a.nq_small && a.q[i] < 1024u; ++i) {
+    const uint32_t q = a.q[i];
+    const uint64_t m = first_multiple(seg_lo, q);
+    const uint64_t off = (m - seg_lo) >> 1;
+    if (off >= nb) continue;
+    for (uint64_t b = off + (uint64_t)threadIdx.x * q; b < nb; b += (uint64_t)blockDim.x * q)
+      atomicOr(&seg[b >> 5], 1u << (b & 31));
+  }
+  for (uint32_t t = i + threadIdx.x; t < a.nq_small; t += blockDim.x) {
+    const uint32_t q = a.q[t];
+    const uint64_t m = first_multiple(seg_lo, q);
+    for (uint64_t b = (m - seg_lo) >> 1; b < nb; b += q) atomicOr(&seg[b >> 5], 1u << (b & 31));
+  }
+  __syncthreads();
+  const uint32_t nw = (nb + 31) / 32;
+  for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+    uint32_t v = seg[w];
+    const uint32_t bb = w * 32;
+    if (bb + 32 > nb) v |= ~((1u << (nb - bb)) - 1u);
+    a.bitmap[b0 / 32 + w] = v;
+  }
+}
+
+// Primes q[nq_small..nq): at most a few multiples per chunk each; one thread
+// per prime, marks with global atomics (after sieve_small_kernel wrote every
+// word of the chunk).
+__global__ __launch_bounds__(256) void sieve_large_kernel(SieveArgs a) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t t = a.nq_small + blockIdx.x * blockDim.x + threadIdx.x; t < a.nq; t += stride) {
+    const uint32_t q = a.q[t];
+    const uint64_t m = first_multiple(a.lo, q);
+    for (uint64_t b = (m - a.lo) >> 1; b < a.nbits; b += q) atomicOr(&a.bitmap[b >> 5], 1u << (b & 31));
+  }
+}
+
+hipError_t launch_sieve(const SieveArgs &a, hipStream_t st) {
+  const uint32_t nseg = (uint32_t)((a.nbits + kSegBits - 1) / kSegBits);
+  hipLaunchKernelGGL(sieve_small_kernel, dim3(nseg), dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (a.nq > a.nq_small) {
+    uint32_t grid = (a.nq - a.nq_small + 255) / 256;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(sieve_large_kernel, dim3(grid), dim3(256), 0, st, a);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// compaction of a chunk bitmap into an ascending prime list
+// ---------------------------------------------------------------------------
+
+// inclusive prefix sum over a 256-thread workgroup; returns the exclusive one
+// and the total through `total`
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t lane = __lane_id();
+  const uint32_t wid = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t j = 0; j < wid; ++j) before += wsum[j];
+  total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void count_kernel(const uint32_t *bitmap, uint32_t nwords, uint32_t *blockcount) {
+  const uint32_t idx = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t c = idx < nwords ? (uint32_t)__popc(~bitmap[idx]) : 0u;
+  uint32_t total;
+  block_exclusive_scan(c, total);
+  if (threadIdx.x == 0) blockcount[blockIdx.x] = total;
+}
+
+// Exclusive scan of blockcount in place, offset by the running count in
+// *count_d, which is then advanced by the chunk's total.  One workgroup.
+__global__ __launch_bounds__(256) void scan_kernel(uint32_t *blockcount, uint32_t nblocks, uint32_t *count_d) {
+  const uint32_t per = (nblocks + 255) / 256;
+  const uint32_t beg = threadIdx.x * per;
+  const uint32_t end = beg + per < nblocks ? beg + per : nblocks;
+  uint32_t local = 0;
+  for (uint32_t j = beg; j < end; ++j) local += blockcount[j];
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan(local, total);
+  const uint32_t base = *count_d;
+  uint32_t run = base + ex;
+  for (uint32_t j = beg; j < end; ++j) {
+    const uint32_t c = blockcount[j];
+    blockcount[j] = run;
+    run += c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *count_d = base + total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, uint32_t nwords, uint64_t lo,
+                                                      const uint32_t *blockoff, T *out,
+                                                      unsigned long long *sum_d) {
+  const uint32_t idx = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t w = idx < nwords ? ~bitmap[idx] : 0u;
+  uint32_t total;
+  uint32_t pos = blockoff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(w), total);
+  unsigned long long s = 0;
+  while (w) {
+    const uint32_t bit = (uint32_t)__builtin_ctz(w);
+    w &= w - 1;
+    const uint64_t x = lo + 2 * ((uint64_t)idx * 32 + bit);
+    out[pos++] = (T)x;
+    s += x;
+  }
+  if (sum_d) {
+    // wave reduction, then one atomic per wave
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (__lane_id() == 0) atomicAdd(sum_d, s);
+  }
+}
+
+template <typename T>
+static hipError_t launch_compact(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, T *out,
+                                 uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
+  const uint32_t nwords = (uint32_t)((nbits + 31) / 32);
+  const uint32_t nblocks = (nwords + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(256), 0, st, bitmap, nwords, blockcount);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, st, blockcount, nblocks, count_d);
+  hipLaunchKernelGGL((compact_kernel<T>), dim3(nblocks), dim3(256), 0, st, bitmap, nwords, lo,
+                     (const uint32_t *)blockcount, out, sum_d);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint64_t *out,
+                            uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
+  return launch_compact<uint64_t>(bitmap, lo, nbits, blockcount, out, count_d, sum_d, st);
+}
+
+hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint32_t *out,
+                            uint32_t *count_d, hipStream_t st) {
+  return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, count_d, nullptr, st);
+}
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void zero_u64_kernel(unsigned long long *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// 2^-e mod p followed by nwin windows of s: the arithmetic alone, for checks
+// against closed forms at any p.
+__global__ __launch_bounds__(256) void state_kernel(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin,
+                                                    uint32_t s, uint64_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t p = P[i];
+  const uint64_t pinv = inv64(p);
+  const uint32_t pinv_sh = lo32(pinv) << (32u - s);
+  uint64_t K = inv2pow(p, pinv, e);
+  for (uint32_t w = 0; w < nwin; ++w) K = step<false>(K, p, pinv_sh, s);
+  out[i] = K >= p ? K - p : K;
+}
+
+hipError_t launch_state(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s, uint64_t *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(state_kernel, dim3((n + 255) / 256), dim3(256), 0, st, P, n, e, nwin, s, out);
+  return hipGetLastError();
+}
+
+}  // namespace ppsd

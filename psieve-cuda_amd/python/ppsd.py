@@ -1,0 +1,254 @@
+"""ctypes binding of libppsd.so (include/ppsd.h): the MI355X PPSieve hot path.
+
+This module is a thin wrapper over the C ABI; all compute runs in the HIP
+library.  Importing it fails loudly when libppsd.so has not been built.
+
+Reference semantics (Ken-g6/PSieve-CUDA): the factor set of a run is every
+``p | k*2^n+c`` with p prime in [pmin, pmax), k == koffset (mod kstep) in
+[kmin, kmax], nmin <= n < nmax, and no prime 3 <= q < 32768 dividing
+k*2^n+c (pps/app.c:150-186, pps/factor_proth.c:246-253); lines are formatted
+as pps/app.c:134 does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libppsd.so")
+
+PPSD_OK = 0
+PPSD_ERR_FAILURE = -1
+PPSD_ERR_INVALID_PARAM = -178
+PPSD_ERR_INSUFFICIENT_RESOURCE = -198
+PPSD_ERR_NOT_IMPLEMENTED = -150
+PPSD_ERR_STATE = -117
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("pmin", C.c_uint64), ("pmax", C.c_uint64),
+        ("kmin", C.c_uint64), ("kmax", C.c_uint64),
+        ("nmin", C.c_uint32), ("nmax", C.c_uint32),
+        ("kstep", C.c_uint32), ("koffset", C.c_uint32),
+        ("sign", C.c_int32),
+        ("nstep_ref", C.c_uint32), ("nstep", C.c_uint32),
+        ("status", C.c_int32), ("msg", C.c_char * 160),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("primes", C.c_uint64), ("prime_sum", C.c_uint64),
+        ("window_steps", C.c_uint64), ("raw_hits", C.c_uint64),
+        ("factors", C.c_uint64), ("walk_launches", C.c_uint64),
+        ("walk_ms", C.c_double), ("sieve_ms", C.c_double), ("filter_ms", C.c_double),
+        ("retries", C.c_uint64),
+    ]
+
+
+FACTOR_DTYPE = np.dtype([("p", "<u8"), ("k", "<u8"), ("n", "<u4"), ("c", "<i4")])
+
+_P = C.POINTER
+_u64p = _P(C.c_uint64)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libppsd.so not built at {LIB_PATH}: run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "ppsd_derive_params": (C.c_int, [_P(Params)]),
+        "ppsd_device_count": (C.c_int, []),
+        "ppsd_open": (C.c_int, [C.c_int, _P(Params), C.c_uint32, _P(C.c_void_p)]),
+        "ppsd_batch_size": (C.c_uint32, [C.c_void_p]),
+        "ppsd_submit_primes": (C.c_int, [C.c_void_p, _u64p, C.c_uint32]),
+        "ppsd_submit_range": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, _u64p]),
+        "ppsd_collect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _u64p, _u64p, _u64p]),
+        "ppsd_collect_raw": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _u64p, _u64p, _u64p]),
+        "ppsd_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+        "ppsd_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
+        "ppsd_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "ppsd_get_stats": (C.c_int, [C.c_void_p, _P(Stats)]),
+        "ppsd_reset_stats": (C.c_int, [C.c_void_p]),
+        "ppsd_close": (C.c_int, [C.c_void_p]),
+        "ppsd_last_error": (C.c_char_p, []),
+        "ppsd_debug_state": (C.c_int, [C.c_int, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+class PPSDError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = lib().ppsd_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        raise PPSDError(rc, where)
+
+
+@dataclass
+class SearchParams:
+    """Options with the reference's meanings; 0 = not given."""
+    pmin: int = 0
+    pmax: int = 0
+    kmin: int = 0
+    kmax: int = 0
+    nmin: int = 0
+    nmax: int = 0
+    kstep: int = 0
+    koffset: int = 0
+    sign: int = 1  # +1 Proth, -1 Riesel (-R), 0 both
+
+    def to_c(self) -> Params:
+        p = Params()
+        for f in ("pmin", "pmax", "kmin", "kmax", "nmin", "nmax", "kstep", "koffset", "sign"):
+            setattr(p, f, getattr(self, f))
+        return p
+
+
+def derive(sp: SearchParams) -> Params:
+    """Apply the reference's parameter rules; raises PPSDError when invalid."""
+    p = sp.to_c()
+    _check(lib().ppsd_derive_params(C.byref(p)), "ppsd_derive_params")
+    return p
+
+
+def device_count() -> int:
+    return lib().ppsd_device_count()
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+class Context:
+    """One device context (the reference's per-thread GPU state)."""
+
+    def __init__(self, params: Params, device: int = 0, batch: int = 0):
+        self._h = C.c_void_p()
+        self.params = params
+        _check(lib().ppsd_open(device, C.byref(params), batch, C.byref(self._h)), "ppsd_open")
+        self.batch = lib().ppsd_batch_size(self._h)
+
+    def close(self):
+        if self._h:
+            lib().ppsd_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, r: int = 0, u: int = 0, wg: int = 0):
+        _check(lib().ppsd_set_tuning(self._h, r, u, wg), "ppsd_set_tuning")
+
+    def tuning(self):
+        r, u, wg = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().ppsd_get_tuning(self._h, C.byref(r), C.byref(u), C.byref(wg)), "ppsd_get_tuning")
+        return r.value, u.value, wg.value
+
+    def submit_primes(self, P: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.uint64)
+        _check(lib().ppsd_submit_primes(self._h, _ptr(P), len(P)), "ppsd_submit_primes")
+
+    def submit_range(self, p0: int, p1: int) -> int:
+        nxt = C.c_uint64()
+        _check(lib().ppsd_submit_range(self._h, p0, p1, C.byref(nxt)), "ppsd_submit_range")
+        return nxt.value
+
+    def _collect(self, fn, where):
+        n, np_, ps = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(fn(self._h, None, 0, C.byref(n), C.byref(np_), C.byref(ps)), where)
+        out = np.zeros(n.value, dtype=FACTOR_DTYPE)
+        if n.value:
+            _check(fn(self._h, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n), C.byref(np_), C.byref(ps)), where)
+        return out, np_.value, ps.value
+
+    def collect(self):
+        """(factors, prime count, prime sum) of the outstanding submission."""
+        return self._collect(lib().ppsd_collect, "ppsd_collect")
+
+    def collect_raw(self):
+        return self._collect(lib().ppsd_collect_raw, "ppsd_collect_raw")
+
+    def set_hit_capacity(self, cap: int):
+        _check(lib().ppsd_set_hit_capacity(self._h, cap), "ppsd_set_hit_capacity")
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(lib().ppsd_get_stats(self._h, C.byref(st)), "ppsd_get_stats")
+        return st
+
+    def reset_stats(self):
+        _check(lib().ppsd_reset_stats(self._h), "ppsd_reset_stats")
+
+    def run_range(self, p0: int, p1: int, raw: bool = False):
+        """Every factor of [p0, p1) with on-device primes; returns
+        (factors, prime count, prime sum)."""
+        parts, cnt, psum = [], 0, 0
+        p = p0
+        while p < p1:
+            nxt = self.submit_range(p, p1)
+            f, c, s = self.collect_raw() if raw else self.collect()
+            parts.append(f)
+            cnt += c
+            psum = (psum + s) & (2**64 - 1)
+            p = nxt
+        out = np.concatenate(parts) if parts else np.zeros(0, dtype=FACTOR_DTYPE)
+        return out, cnt, psum
+
+    def run_primes(self, P: np.ndarray, raw: bool = False):
+        """Every factor for host-supplied primes, batch by batch."""
+        P = np.ascontiguousarray(P, dtype=np.uint64)
+        parts = []
+        for i in range(0, len(P), self.batch):
+            self.submit_primes(P[i:i + self.batch])
+            f, _, _ = self.collect_raw() if raw else self.collect()
+            parts.append(f)
+        return np.concatenate(parts) if parts else np.zeros(0, dtype=FACTOR_DTYPE)
+
+
+def debug_state(P, e: int, nwin: int, s: int, device: int = 0) -> np.ndarray:
+    """2^-(e + nwin*s) mod P[i] by the device arithmetic."""
+    P = np.ascontiguousarray(P, dtype=np.uint64)
+    out = np.zeros_like(P)
+    _check(lib().ppsd_debug_state(device, _ptr(P), len(P), e, nwin, s, _ptr(out)), "ppsd_debug_state")
+    return out
+
+
+def format_factor(f) -> str:
+    """One ppfactors.txt line, "%llu | %llu*2^%u%+d" (pps/app.c:134)."""
+    return f"{int(f['p'])} | {int(f['k'])}*2^{int(f['n'])}{int(f['c']):+d}"
+
+
+def factor_set(arr) -> set:
+    return {(int(x["p"]), int(x["k"]), int(x["n"]), int(x["c"])) for x in arr}

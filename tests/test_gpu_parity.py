@@ -1,0 +1,176 @@
+"""Parity of the HIP path (libppsd.so through its C ABI) with the oracle and
+the golden fixtures.  Integer work: every comparison is exact set equality."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+import sympy
+
+import pps_oracle as orc
+import ppsd
+from conftest import GOLDEN, read_lines
+
+pytestmark = pytest.mark.gpu
+
+RUNS = json.load(open(os.path.join(GOLDEN, "runs.json")))
+MANY_N = dict(pmin=42070000000000, pmax=42070050000000, kmin=1201, kmax=9999, nmax=2000000)
+
+
+def golden_set(name):
+    return {orc.parse_line(l) for l in read_lines(name)}
+
+
+def context(batch=0, **kw):
+    return ppsd.Context(ppsd.derive(ppsd.SearchParams(**kw)), batch=batch)
+
+
+def test_device_present():
+    assert ppsd.device_count() >= 1
+
+
+def test_state_vectors_against_pow():
+    """2^-(e + w*s) mod p from the device arithmetic, p across 2^32..2^62."""
+    rng = random.Random(5)
+    P = [sympy.nextprime(rng.randrange(2**32, 2**62)) for _ in range(300)]
+    P += [sympy.nextprime(2**32), sympy.prevprime(2**62), sympy.nextprime(2**50), sympy.nextprime(2**61),
+          sympy.nextprime(10**12)]
+    P = np.array(P, dtype=np.uint64)
+    for e, nwin, s in [(64, 0, 32), (76, 3, 32), (100, 7404, 27), (50000, 1000, 32), (100000, 517, 32),
+                       (2**31 - 1, 5, 1), (6000000, 31, 17), (1000, 2000, 31)]:
+        got = ppsd.debug_state(P, e, nwin, s)
+        want = [pow(2, -(e + nwin * s), int(p)) for p in P]
+        assert got.tolist() == want, (e, nwin, s)
+
+
+def test_many_n_raw_and_filtered():
+    """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
+    pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
+    sum as the oracle's."""
+    with context(**MANY_N) as ctx:
+        raw, cnt, psum = ctx.run_range(MANY_N["pmin"], MANY_N["pmax"], raw=True)
+        assert ppsd.factor_set(raw) == golden_set("many_n_raw.txt")
+        assert len(raw) == 296
+        assert (cnt, psum) == (RUNS["many_n_raw"]["count"], RUNS["many_n_raw"]["sum"])
+        got, _, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmax"])
+        assert ppsd.factor_set(got) == golden_set("many_n_filtered.txt")
+
+
+@pytest.mark.parametrize("r,u,wg", [(1, 4, 256), (1, 8, 128), (1, 16, 256), (2, 4, 64), (2, 8, 256),
+                                    (2, 16, 192)])
+def test_many_n_every_kernel_shape(r, u, wg):
+    with context(**MANY_N) as ctx:
+        ctx.set_tuning(r, u, wg)
+        raw, _, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmin"] + 10**7, raw=True)
+        want = {t for t in golden_set("many_n_raw.txt") if t[0] < MANY_N["pmin"] + 10**7}
+        assert ppsd.factor_set(raw) == want
+
+
+def test_many_n_host_primes():
+    """check_ns-style path: primes supplied by the host, several batches."""
+    lo, hi = MANY_N["pmin"], MANY_N["pmin"] + 5 * 10**6
+    P = orc.primes(lo, hi)
+    with context(batch=50000, **MANY_N) as ctx:
+        raw = ctx.run_primes(P, raw=True)
+    want = {t for t in golden_set("many_n_raw.txt") if t[0] < hi}
+    assert ppsd.factor_set(raw) == want
+
+
+def test_hit_ring_overflow_rewalks():
+    with context(**MANY_N) as ctx:
+        ctx.set_hit_capacity(7)
+        raw, _, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmax"], raw=True)
+        assert ppsd.factor_set(raw) == golden_set("many_n_raw.txt")
+        assert ctx.stats().retries > 0
+
+
+def _golden_run(tag, name):
+    r = RUNS[tag]
+    kw = dict(r["params"])
+    with context(**kw) as ctx:
+        got, cnt, psum = ctx.run_range(r["params"]["pmin"], r["params"]["pmax"])
+        st = ctx.stats()
+    assert ppsd.factor_set(got) == golden_set(name)
+    assert (cnt, psum) == (r["count"], r["sum"])
+    return st
+
+
+def test_cfg1_full_range():
+    """-p1e12 -P1000100M -k3 -K9999 -n100 -N200000 (s = 27), every factor."""
+    st = _golden_run("cfg1", "cfg1_factors.txt")
+    assert st.primes == RUNS["cfg1"]["count"]
+
+
+def test_cfg2_slice():
+    _golden_run("cfg2_slice", "cfg2_slice.txt")
+
+
+def test_cfg3_slice_both_signs():
+    _golden_run("cfg3_slice", "cfg3_slice.txt")
+
+
+def _random_config(rng):
+    pmin = rng.choice([rng.randrange(2**33, 2**36), rng.randrange(2**36, 2**44)])
+    kmax = rng.choice([rng.randrange(3, 200), rng.randrange(200, 20000), rng.randrange(20000, 2**22)])
+    kmax = min(kmax, pmin - 1)
+    kmin = rng.randrange(1, kmax + 1)
+    nmin = rng.randrange(64, 5000)
+    nmax = nmin + rng.randrange(0, 6000)
+    sign = rng.choice((1, -1, 0))
+    ks = rng.choice([(0, 0), (0, 0), (6, 3)])
+    return dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 60000), kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax,
+                sign=sign, kstep=ks[0], koffset=ks[1])
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_configs_vs_oracle(seed):
+    rng = random.Random(1000 + seed)
+    kw = _random_config(rng)
+    pr = orc.derive(**kw)
+    if pr.status != 0:
+        with pytest.raises(ppsd.PPSDError):
+            ppsd.derive(ppsd.SearchParams(**kw))
+        return
+    P = orc.primes(pr.pmin, pr.pmax)
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+    want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
+    with context(**kw) as ctx:
+        ctx.set_tuning(r=1 + seed % 2, u=(4, 8, 16)[seed % 3])
+        raw, cnt, _ = ctx.run_range(pr.pmin, pr.pmax, raw=True)
+        got, _, _ = ctx.run_range(pr.pmin, pr.pmax)
+    assert cnt == len(P)
+    assert ppsd.factor_set(raw) == want_raw, kw
+    assert ppsd.factor_set(got) == want, kw
+
+
+def test_empty_inputs():
+    kw = dict(MANY_N, nmin=76, nmax=76)   # nmin == nmax: no exponent in range
+    with context(**kw) as ctx:
+        got, cnt, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmin"] + 10**6)
+        assert len(got) == 0 and cnt > 0
+    with context(**MANY_N) as ctx:
+        assert ctx.submit_range(MANY_N["pmin"], MANY_N["pmin"]) == MANY_N["pmin"]
+        got, cnt, _ = ctx.collect()
+        assert len(got) == 0 and cnt == 0
+        ctx.submit_primes(np.zeros(0, dtype=np.uint64))
+        got, cnt, _ = ctx.collect()
+        assert len(got) == 0 and cnt == 0
+
+
+def test_errors():
+    with context(**MANY_N) as ctx:
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.submit_primes(np.array([MANY_N["pmin"] + 1], dtype=np.uint64))  # even
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.submit_range(MANY_N["pmin"] - 10, MANY_N["pmin"] + 10)
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+        ctx.submit_range(MANY_N["pmin"], MANY_N["pmin"] + 1000)
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.submit_range(MANY_N["pmin"], MANY_N["pmin"] + 1000)
+        assert e.value.code == ppsd.PPSD_ERR_STATE
+        ctx.collect()
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.set_tuning(r=3)
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
