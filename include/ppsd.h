@@ -78,7 +78,11 @@ int ppsd_derive_params(ppsd_params *p);
 int ppsd_device_count(void);
 
 /* Per-worker device set-up; replaces cuda_app_init (pps/appcu.h:27,
- * pps/appcu.cu:213-404).  p must have passed ppsd_derive_params.  batch is
+ * pps/appcu.cu:213-404).  p normally comes from ppsd_derive_params; any p
+ * with status 0, 2^32 < pmin < pmax <= 2^62, 64 <= nmin <= nmax,
+ * 1 <= kmin <= kmax, 1 <= nstep <= 32 and kmax * 2^(nstep-1) < pmin (one
+ * candidate per window) is accepted, which also admits work units outside
+ * the reference's own validation (2^(64-nstep) <= pmin).  batch is
  * the number of primes per walk launch (0 = default: 32 per SIMD lane slot of
  * the device, i.e. 2^21 on 256 CUs).  Allocates every device buffer up
  * front; generates the sieving primes <= isqrt(pmax-1) on the device. */

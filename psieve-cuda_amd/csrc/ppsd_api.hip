@@ -212,7 +212,8 @@ static int build_sieving_primes(ppsd_ctx *c) {
     sa.nq = (uint32_t)base.size();
     sa.bitmap = c->d_bitmap;
     HIP_TRY(launch_sieve(sa, c->stream));
-    HIP_TRY(launch_compact32(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_sp, c->d_count, c->stream));
+    HIP_TRY(launch_compact32(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_sp, (uint32_t)cap, c->d_count,
+                             c->stream));
     lo += 2 * nbits;
   }
   uint32_t n = 0;
@@ -232,8 +233,14 @@ static int build_sieving_primes(ppsd_ctx *c) {
 int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) {
   if (!p || !out) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
   *out = nullptr;
-  if (p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) || p->nmin < 64)
-    return set_err(PPSD_ERR_INVALID_PARAM, "parameters have not passed ppsd_derive_params");
+  // What the walk itself needs (ppsd_derive_params gives a subset of this):
+  // one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
+  // nmin >= 64 for the 2^-nmin set-up.
+  if (p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) ||
+      p->pmax > ((uint64_t)1 << 62) || p->pmin >= p->pmax || p->nmin < 64 || p->nmin > p->nmax || p->kmin < 1 ||
+      p->kmin > p->kmax || (p->kmax >> (64 - p->nstep)) != 0 || (p->kmax << (p->nstep - 1)) >= p->pmin ||
+      p->sign < -1 || p->sign > 1 || p->kstep < 1 || p->koffset >= p->kstep)
+    return set_err(PPSD_ERR_INVALID_PARAM, "parameters outside the walk's domain (use ppsd_derive_params)");
   int ndev = ppsd_device_count();
   if (device < 0 || device >= ndev) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d not available (%d devices)", device, ndev);
   ppsd_ctx *c = new ppsd_ctx;
@@ -335,6 +342,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   a.P = c->d_P + first;
   a.count_d = count_d;
   a.count = n;
+  a.cap = count_d ? c->batch : n;
   a.nmin = c->prm.nmin;
   a.nmax = c->prm.nmax;
   a.s = c->prm.nstep;
@@ -428,8 +436,8 @@ int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
     sa.bitmap = c->d_bitmap;
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(launch_sieve(sa, c->stream));
-    HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->d_count, c->d_counters + 2,
-                             c->stream));
+    HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->batch, c->d_count,
+                             c->d_counters + 2, c->stream));
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   }
   int rc = enqueue_walk(c, 0, 0, c->d_count, c->batch);

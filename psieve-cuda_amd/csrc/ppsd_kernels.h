@@ -19,6 +19,7 @@ struct WalkArgs {
   const uint64_t *P;        // primes, any order
   const uint32_t *count_d;  // prime count on the device (range mode) or null
   uint32_t count;           // prime count when count_d is null
+  uint32_t cap;             // entries of P: the device count is clamped to it
   uint32_t nmin, nmax;      // nmax exclusive
   uint32_t s;               // window, 1..32
   uint32_t nwin;            // windows to walk: ceil((nmax - nmin) / s)
@@ -62,11 +63,13 @@ hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid,
 hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st);
 hipError_t launch_sieve(const SieveArgs &a, hipStream_t st);
 // Compaction: primes of the chunk to out (ascending), appended at *count_d;
-// *sum_d accumulates their sum mod 2^64.
+// at most cap entries of out are written, *count_d counts them all; *sum_d
+// accumulates their sum mod 2^64.
 hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
-                            uint64_t *out, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st);
+                            uint64_t *out, uint32_t cap, uint32_t *count_d, unsigned long long *sum_d,
+                            hipStream_t st);
 hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
-                            uint32_t *out, uint32_t *count_d, hipStream_t st);
+                            uint32_t *out, uint32_t cap, uint32_t *count_d, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);
 hipError_t launch_state(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s, uint64_t *out,

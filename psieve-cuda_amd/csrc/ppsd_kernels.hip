@@ -94,7 +94,8 @@ __device__ __noinline__ void rescan(const uint64_t (&p)[R], const uint32_t (&pin
 // leaves the straight-line path when one of its pre-tests fired.
 template <int SIGN, int R, int U, bool S32>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
-  const uint32_t count = a.count_d ? *a.count_d : a.count;
+  const uint32_t cnt = a.count_d ? *a.count_d : a.count;
+  const uint32_t count = cnt < a.cap ? cnt : a.cap;
   const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * R;
   if (first >= count) return;
   uint64_t p[R], K[R];
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(256) void scan_kernel(uint32_t *blockcount, uint32_
 
 template <typename T>
 __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, uint32_t nwords, uint64_t lo,
-                                                      const uint32_t *blockoff, T *out,
+                                                      const uint32_t *blockoff, T *out, uint32_t cap,
                                                       unsigned long long *sum_d) {
   const uint32_t idx = blockIdx.x * kScanBlock + threadIdx.x;
   uint32_t w = idx < nwords ? ~bitmap[idx] : 0u;
@@ -407,7 +408,8 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, ui
     const uint32_t bit = (uint32_t)__builtin_ctz(w);
     w &= w - 1;
     const uint64_t x = lo + 2 * ((uint64_t)idx * 32 + bit);
-    out[pos++] = (T)x;
+    if (pos < cap) out[pos] = (T)x;  // the count still advances: the host sees an overflow
+    ++pos;
     s += x;
   }
   if (sum_d) {
@@ -419,24 +421,24 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, ui
 
 template <typename T>
 static hipError_t launch_compact(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, T *out,
-                                 uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
+                                 uint32_t cap, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
   const uint32_t nwords = (uint32_t)((nbits + 31) / 32);
   const uint32_t nblocks = (nwords + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(256), 0, st, bitmap, nwords, blockcount);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, st, blockcount, nblocks, count_d);
   hipLaunchKernelGGL((compact_kernel<T>), dim3(nblocks), dim3(256), 0, st, bitmap, nwords, lo,
-                     (const uint32_t *)blockcount, out, sum_d);
+                     (const uint32_t *)blockcount, out, cap, sum_d);
   return hipGetLastError();
 }
 
 hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint64_t *out,
-                            uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
-  return launch_compact<uint64_t>(bitmap, lo, nbits, blockcount, out, count_d, sum_d, st);
+                            uint32_t cap, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
+  return launch_compact<uint64_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, sum_d, st);
 }
 
 hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint32_t *out,
-                            uint32_t *count_d, hipStream_t st) {
-  return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, count_d, nullptr, st);
+                            uint32_t cap, uint32_t *count_d, hipStream_t st) {
+  return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, nullptr, st);
 }
 
 // ---------------------------------------------------------------------------

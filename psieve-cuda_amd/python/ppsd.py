@@ -134,6 +134,13 @@ def derive(sp: SearchParams) -> Params:
     return p
 
 
+def manual_params(pmin, pmax, kmin, kmax, nmin, nmax, sign=1, nstep=32, kstep=2, koffset=1) -> Params:
+    """Parameters set directly (no reference validation), for work units the
+    walk supports but the reference rejects; ppsd_open checks the domain."""
+    return Params(pmin=pmin, pmax=pmax, kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax, kstep=kstep,
+                  koffset=koffset, sign=sign, nstep_ref=nstep, nstep=nstep, status=0)
+
+
 def device_count() -> int:
     return lib().ppsd_device_count()
 

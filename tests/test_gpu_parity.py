@@ -111,19 +111,35 @@ def test_cfg3_slice_both_signs():
 
 
 def _random_config(rng):
-    pmin = rng.choice([rng.randrange(2**33, 2**36), rng.randrange(2**36, 2**44)])
-    kmax = rng.choice([rng.randrange(3, 200), rng.randrange(200, 20000), rng.randrange(20000, 2**22)])
-    kmax = min(kmax, pmin - 1)
+    """A config inside the reference's validation domain."""
+    pmin = rng.choice([rng.randrange(2**40, 2**44), rng.randrange(2**44, 2**50)])
+    kmax = rng.randrange(3, min(2**22, pmin * pmin >> 63))
     kmin = rng.randrange(1, kmax + 1)
     nmin = rng.randrange(64, 5000)
-    nmax = nmin + rng.randrange(0, 6000)
+    nmax = nmin + rng.randrange(0, 400000)
     sign = rng.choice((1, -1, 0))
     ks = rng.choice([(0, 0), (0, 0), (6, 3)])
-    return dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 60000), kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax,
+    return dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 40000), kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax,
                 sign=sign, kstep=ks[0], koffset=ks[1])
 
 
-@pytest.mark.parametrize("seed", range(24))
+def _compare(pr_orc, params, seed, batch=0):
+    P = orc.primes(pr_orc.pmin, pr_orc.pmax)
+    want_raw = orc.hit_set(orc.walk(pr_orc, P, filter=False, threads=8))
+    want = orc.hit_set(orc.walk(pr_orc, P, filter=True, threads=8))
+    with ppsd.Context(params, batch=batch) as ctx:
+        ctx.set_tuning(r=1 + seed % 2, u=(4, 8, 16)[seed % 3])
+        raw, cnt, _ = ctx.run_range(pr_orc.pmin, pr_orc.pmax, raw=True)
+        got, _, _ = ctx.run_range(pr_orc.pmin, pr_orc.pmax)
+        host = ctx.run_primes(P, raw=True)
+    assert cnt == len(P)
+    assert ppsd.factor_set(raw) == want_raw
+    assert ppsd.factor_set(host) == want_raw
+    assert ppsd.factor_set(got) == want
+    return len(want_raw)
+
+
+@pytest.mark.parametrize("seed", range(16))
 def test_random_configs_vs_oracle(seed):
     rng = random.Random(1000 + seed)
     kw = _random_config(rng)
@@ -132,21 +148,35 @@ def test_random_configs_vs_oracle(seed):
         with pytest.raises(ppsd.PPSDError):
             ppsd.derive(ppsd.SearchParams(**kw))
         return
-    P = orc.primes(pr.pmin, pr.pmax)
-    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
-    want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
-    with context(**kw) as ctx:
-        ctx.set_tuning(r=1 + seed % 2, u=(4, 8, 16)[seed % 3])
-        raw, cnt, _ = ctx.run_range(pr.pmin, pr.pmax, raw=True)
-        got, _, _ = ctx.run_range(pr.pmin, pr.pmax)
-    assert cnt == len(P)
-    assert ppsd.factor_set(raw) == want_raw, kw
-    assert ppsd.factor_set(got) == want, kw
+    _compare(pr, ppsd.derive(ppsd.SearchParams(**kw)), seed)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_dense_hit_configs_vs_oracle(seed):
+    """Units outside the reference's validation (small p for their kmax) where
+    most lanes of a wave hit: compaction, ring and filter under load."""
+    rng = random.Random(2000 + seed)
+    pmin = rng.randrange(2**32 + 1, 2**36)
+    s = rng.randrange(1, 33)
+    kmax = max(3, min((pmin - 1) >> (s - 1), rng.randrange(3, 2**24)))
+    while kmax << (s - 1) >= pmin:
+        kmax >>= 1
+    kmin = rng.randrange(1, kmax + 1)
+    nmin = rng.randrange(64, 3000)
+    nmax = nmin + rng.randrange(1, 20000)
+    sign = rng.choice((1, -1, 0))
+    pmax = pmin + rng.randrange(500, 20000)
+    kw = dict(pmin=pmin, pmax=pmax, kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax, sign=sign, nstep=s)
+    pr = orc.OrcParams(kstep=2, koffset=1, nstep_ref=s, **kw)
+    params = ppsd.manual_params(**kw)
+    n = _compare(pr, params, seed, batch=rng.choice([0, 257, 4096]))
+    assert n > 0
 
 
 def test_empty_inputs():
-    kw = dict(MANY_N, nmin=76, nmax=76)   # nmin == nmax: no exponent in range
-    with context(**kw) as ctx:
+    # nmin == nmax: no exponent in range (the reference rejects it: nstep 1)
+    params = ppsd.manual_params(MANY_N["pmin"], MANY_N["pmax"], 1201, 9999, 76, 76, nstep=32)
+    with ppsd.Context(params) as ctx:
         got, cnt, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmin"] + 10**6)
         assert len(got) == 0 and cnt > 0
     with context(**MANY_N) as ctx:
