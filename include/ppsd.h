@@ -35,6 +35,7 @@ extern "C" {
 typedef struct ppsd_params {
   uint64_t pmin, pmax;     /* -p, -P */
   uint64_t kmin, kmax;     /* -k, -K */
+  uint64_t qmax;           /* -Q: sieve p with odd primes <= qmax only (0 = all <= sqrt) */
   uint32_t nmin, nmax;     /* -n, -N; nmax is exclusive */
   uint32_t kstep, koffset; /* -M, -s; kstep 0 selects 2 / 1 (odd k) */
   int32_t sign;            /* +1 Proth, -1 Riesel (-R), 0 both */

@@ -282,7 +282,11 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
     OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(uint32_t)));
     OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
+  // Sieving primes: every odd q <= isqrt(pmax-1), or only those <= -Q when
+  // given (main.c:971-973; primes up to 61 are always used, sieve.c's
+  // presieve: sieve.c:770-773).
   c->qmax = isqrt64(p->pmax - 1);
+  if (p->qmax) c->qmax = std::min<uint64_t>(c->qmax, std::max<uint64_t>(p->qmax, 61));
   rc = build_sieving_primes(c);
   if (rc) return bail(rc);
 #undef OPEN_TRY

@@ -1,0 +1,263 @@
+/* pps_app.c -- the Proth/Riesel application behind the PPSieve hooks,
+ * running its hot path on an MI355X through libppsd (include/ppsd.h).
+ *
+ * Hook contract: pps/app.h:73-84 as main.c calls them.  What changes from
+ * the reference's pps/app.c is where the work happens: the device returns
+ * exact, already filtered factors (no per-prime flag byte and no host
+ * re-walk, pps/app.c:1197-1219), and app_thread_range lets the device
+ * generate the primes itself.
+ */
+#include <getopt.h>
+#include <inttypes.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include "../../include/ppsd.h"
+#include "pps_app.h"
+
+uint64_t kmin = 0, kmax = 0;
+unsigned int nmin = 0, nmax = 0;
+int search_proth = 1; /* +1 Proth, -1 Riesel (-R), 0 both (--both) */
+
+static const char *input_filename = NULL;
+static const char *factors_filename = NULL;
+static FILE *factors_file = NULL;
+static unsigned int factor_count = 0;
+static int print_factors = 1;
+static int device_opt = -1;
+static unsigned int user_cthread_count = 0;
+static unsigned int kstep = 2, koffset = 1;
+static ppsd_params params;
+static pthread_mutex_t factors_mutex = PTHREAD_MUTEX_INITIALIZER;
+
+#define MAX_WORKERS 32
+static ppsd_ctx *ctxs[MAX_WORKERS];
+static int pending[MAX_WORKERS]; /* a submission is on the device */
+static ppsd_factor *fbuf[MAX_WORKERS];
+static uint64_t fcap[MAX_WORKERS];
+
+/* One line per factor, "%llu | %llu*2^%u%+d" (pps/app.c:124-148). */
+static void report_factor(const ppsd_factor *f) {
+  pthread_mutex_lock(&factors_mutex);
+  if (factors_file != NULL &&
+      fprintf(factors_file, "%" PRIu64 " | %" PRIu64 "*2^%u%+d\n", f->p, f->k, f->n, f->c) > 0) {
+    if (print_factors) printf("%" PRIu64 " | %" PRIu64 "*2^%u%+d\n", f->p, f->k, f->n, f->c);
+  } else
+    fprintf(stderr, "%sUNSAVED: %" PRIu64 " | %" PRIu64 "*2^%u%+d\n", bmprefix(), f->p, f->k, f->n, f->c);
+  factor_count++;
+  pthread_mutex_unlock(&factors_mutex);
+}
+
+static void device_fail(int th, const char *where, int rc) {
+  fprintf(stderr, "%sDevice error (thread %d): %s: %s\n", bmprefix(), th, where, ppsd_last_error());
+  bexit(rc == PPSD_ERR_INSUFFICIENT_RESOURCE ? ERR_INSUFFICIENT_RESOURCE : EXIT_FAILURE);
+}
+
+/* Wait for thread th's submission and report its factors. */
+static int drain(int th, uint64_t *count, uint64_t *sum) {
+  uint64_t n = 0, cnt = 0, s = 0;
+  int rc;
+  if (!pending[th]) return 0;
+  pending[th] = 0;
+  rc = ppsd_collect(ctxs[th], fbuf[th], fcap[th], &n, &cnt, &s);
+  if (rc) return rc;
+  if (n > fcap[th]) {
+    free(fbuf[th]);
+    fcap[th] = n + n / 2;
+    fbuf[th] = malloc(fcap[th] * sizeof(ppsd_factor));
+    if (!fbuf[th]) return PPSD_ERR_INSUFFICIENT_RESOURCE;
+    rc = ppsd_collect(ctxs[th], fbuf[th], fcap[th], &n, &cnt, &s);
+    if (rc) return rc;
+  }
+  for (uint64_t i = 0; i < n; i++) report_factor(&fbuf[th][i]);
+  if (count) *count += cnt;
+  if (sum) *sum += s;
+  return 0;
+}
+
+void app_banner(void) { printf(APP_NAME " version " APP_VERSION " (MI355X, gfx950)\n"); }
+
+/* Option meanings of pps/app.c:551-617; returns 0, -1 invalid, -2 range. */
+int app_parse_option(int opt, char *arg, const char *source) {
+  int status = 0;
+  switch (opt) {
+    case 'k': status = parse_uint64(&kmin, arg, 1, (UINT64_C(1) << 62) - 1); break;
+    case 'K': status = parse_uint64(&kmax, arg, 1, (UINT64_C(1) << 62) - 1); break;
+    case 'm': status = parse_uint(&user_cthread_count, arg, 1, 1U << 31); break;
+    case 'n': status = parse_uint(&nmin, arg, 1, (1U << 31) - 1); break;
+    case 'N': status = parse_uint(&nmax, arg, 1, (1U << 31) - 1); break;
+    case 'i': input_filename = (source == NULL) ? arg : strdup(arg); break;
+    case 'f': factors_filename = (source == NULL) ? arg : strdup(arg); break;
+    case 'd': status = parse_uint((unsigned int *)&device_opt, arg, 0, INT32_MAX); break;
+    case 'M':
+      status = parse_uint(&kstep, arg, 1, (1U << 31) - 1);
+      if (koffset >= kstep) koffset = kstep / 2;
+      break;
+    case 's': status = parse_uint(&koffset, arg, 1, (1U << 31) - 1); break;
+    case 'R': search_proth = -1; break;
+    case 257: search_proth = 0; break;
+    case 'b': break; /* accepted and ignored, as pps/app.c:557-561 */
+  }
+  return status;
+}
+
+void app_help(void) {
+  printf("-f --factors=FILE  Write factors to FILE (default `%s')\n", FACTORS_FILENAME_DEFAULT);
+  printf("-i --input=FILE    Read initial sieve from FILE (not supported by this build)\n");
+  printf("-M --modulus=M     Print only k's == s mod M (default 2)\n");
+  printf("-s --modshift=S    (default 1, or M/2)\n");
+  printf("-k --kmin=K0\n");
+  printf("-K --kmax=K1       Sieve for primes k*2^n+/-1 with K0 <= k <= K1\n");
+  printf("-m --mthreads=M    Primes per device batch = M per compute unit\n");
+  printf("-n --nmin=N0\n");
+  printf("-N --nmax=N1       Sieve for primes k*2^n+/-1 with N0 <= n < N1\n");
+  printf("-R --riesel        Sieve for primes k*2^n-1 instead of +1.\n");
+  printf("   --both          Sieve k*2^n+1 and k*2^n-1 in one pass.\n");
+  printf("-d --device=N      Use GPU N instead of 0-threads\n");
+}
+
+/* Parameter rules and the factors file (pps/app.c:649-790): the rules are
+ * ppsd_derive_params'; failures exit with the reference's code and text. */
+void app_init(void) {
+  int rc;
+  print_factors = quiet_opt ? 0 : 1;
+  if (input_filename != NULL) {
+    bmsg("Input sieve files (-i) are not supported by this build\n");
+    bexit(ERR_NOT_IMPLEMENTED);
+  }
+  memset(&params, 0, sizeof params);
+  params.pmin = pmin;
+  params.pmax = pmax;
+  params.kmin = kmin;
+  params.kmax = kmax;
+  params.nmin = nmin;
+  params.nmax = nmax;
+  params.kstep = kstep;
+  params.koffset = koffset;
+  params.sign = search_proth;
+  params.qmax = qmax_opt;
+  rc = ppsd_derive_params(&params);
+  if (rc) {
+    char buf[200];
+    snprintf(buf, sizeof buf, "%s\n", params.msg);
+    bmsg(buf);
+    bexit(rc == PPSD_ERR_FAILURE ? EXIT_FAILURE : rc);
+  }
+  kmin = params.kmin;
+  kmax = params.kmax;
+  nmin = params.nmin;
+  nmax = params.nmax;
+  printf("nstart=%u, nstep=%u\n", nmin, params.nstep_ref);
+  if (params.nstep != params.nstep_ref) printf("nstep changed to %u\n", params.nstep);
+  if (factors_filename == NULL) factors_filename = FACTORS_FILENAME_DEFAULT;
+  if ((factors_file = fopen(factors_filename, "a")) == NULL) {
+    fprintf(stderr, "%sCannot open factors file `%s'\n", bmprefix(), factors_filename);
+    bexit(ERR_FOPEN);
+  }
+  printf(APP_NAME " initialized: %" PRIu64 " <= k <= %" PRIu64 ", %u <= n < %u\n", kmin, kmax, nmin, nmax);
+  fflush(stdout);
+}
+
+/* Per worker: open the device (pps/app.c:797-827 -> cuda_app_init).
+ * Returns the batch size, 0 when the device cannot be used. */
+unsigned int app_thread_init(int th) {
+  const int dev = device_opt >= 0 ? device_opt : th;
+  uint32_t batch = 0;
+  int rc;
+  if (th < 0 || th >= MAX_WORKERS) return 0;
+  if (user_cthread_count) batch = user_cthread_count * 256u;
+  rc = ppsd_open(dev, &params, batch, &ctxs[th]);
+  if (rc) {
+    fprintf(stderr, "%sGPU %d not usable: %s\n", bmprefix(), dev, ppsd_last_error());
+    if (rc == PPSD_ERR_INVALID_PARAM) bexit(ERR_INVALID_PARAM);
+    return 0;
+  }
+  fcap[th] = 1024;
+  fbuf[th] = malloc(fcap[th] * sizeof(ppsd_factor));
+  fprintf(stderr, "%sThread %d: GPU %d, %u primes per batch\n", bmprefix(), th, dev, ppsd_batch_size(ctxs[th]));
+  return ppsd_batch_size(ctxs[th]);
+}
+
+/* A batch of host primes (pps/app.c:1224-1250): finish the previous batch,
+ * start this one, return without waiting.  The device copies P, so lastP
+ * is not needed to remember the batch in flight. */
+void app_thread_fun(int th, const uint64_t *P, uint64_t *lastP, const unsigned int cthread_count) {
+  int rc = drain(th, NULL, NULL);
+  if (rc) device_fail(th, "collect", rc);
+  rc = ppsd_submit_primes(ctxs[th], P, cthread_count);
+  if (rc) device_fail(th, "submit_primes", rc);
+  pending[th] = 1;
+}
+
+/* Flush (pps/app.c:1257-1286): the last len primes, then drain everything,
+ * so that a checkpoint may follow.  No padding is needed: a batch may be
+ * shorter than cthread_count. */
+void app_thread_fun1(int th, uint64_t *P, uint64_t *lastP, const unsigned int cthread_count, unsigned int len) {
+  int rc;
+  if (len > 0) app_thread_fun(th, P, lastP, len);
+  rc = drain(th, NULL, NULL);
+  if (rc) device_fail(th, "collect", rc);
+}
+
+int app_thread_range(int th, uint64_t p0, uint64_t p1, uint64_t *count, uint64_t *sum) {
+  uint64_t p = p0;
+  int rc;
+  *count = 0;
+  *sum = 0;
+  while (p < p1) {
+    uint64_t next = p1;
+    rc = ppsd_submit_range(ctxs[th], p, p1, &next);
+    if (rc) return rc;
+    pending[th] = 1;
+    rc = drain(th, count, sum);
+    if (rc) return rc;
+    p = next;
+  }
+  return 0;
+}
+
+void app_thread_fini(int th) {
+  ppsd_close(ctxs[th]);
+  ctxs[th] = NULL;
+  free(fbuf[th]);
+  fbuf[th] = NULL;
+}
+
+/* Second checkpoint line, "nmin=%u,nmax=%u,factor_count=%u"
+ * (pps/app.c:1301-1352). */
+int app_read_checkpoint(FILE *fin) {
+  unsigned int n0, n1;
+  if (fscanf(fin, "nmin=%u,nmax=%u,factor_count=%u", &n0, &n1, &factor_count) != 3) {
+    factor_count = 0;
+    return 0;
+  }
+  if (n0 != nmin || n1 != nmax) {
+    factor_count = 0;
+    return 0;
+  }
+  if (factor_count > 0) {
+    struct stat st;
+    fstat(fileno(factors_file), &st);
+    if (st.st_size == 0) {
+      bmsg("Factors file missing.  Restarting test.\n");
+      factor_count = 0;
+      return 0;
+    }
+  }
+  return 1;
+}
+
+void app_write_checkpoint(FILE *fout) {
+  fflush(factors_file);
+  fprintf(fout, "nmin=%u,nmax=%u,factor_count=%u\n", nmin, nmax, factor_count);
+}
+
+void app_fini(uint64_t pstop) {
+  (void)pstop;
+  fclose(factors_file);
+  factors_file = NULL;
+  printf("Found %u factor%s\n", factor_count, (factor_count == 1) ? "" : "s");
+}

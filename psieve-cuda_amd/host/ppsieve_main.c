@@ -1,0 +1,615 @@
+/* ppsieve_main.c -- the framework around the PPSieve application hooks:
+ * options and ppconfig.txt, worker threads (one per GPU), checkpoints, status
+ * reports and the closing statistics.
+ *
+ * Behaviour follows the reference's main.c (option names and meanings
+ * main.c:375-650, checkpoint file main.c:305-372, final lines
+ * main.c:1196-1257).  The prime supply is the difference: by default every
+ * worker asks the device for all primes of a p-chunk (app_thread_range), so
+ * no prime crosses PCIe; --host-primes instead runs a host segmented sieve
+ * and feeds batches through app_thread_fun/app_thread_fun1 exactly as
+ * main.c:736-832 does.
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <getopt.h>
+#include <inttypes.h>
+#include <math.h>
+#include <pthread.h>
+#include <signal.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/resource.h>
+#include <sys/time.h>
+#include <time.h>
+
+#include "pps_app.h"
+
+#define MAX_THREADS 32
+#define PMIN_MIN 3
+#define PMAX_MAX (UINT64_C(1) << 62)
+#define QMAX_MAX (1U << 31)
+
+int num_threads = 1;
+uint64_t pmin = 0, pmax = 0, pstart = 0;
+unsigned int quiet_opt = 0;
+unsigned int qmax_opt = 0;
+
+static char *pmin_str = NULL;
+static char *checkpoint_filename = NULL;
+static unsigned int report_opt = 60;       /* main.h:44 */
+static unsigned int checkpoint_opt = 300;  /* main.h:45 */
+static unsigned int priority_opt = 0;
+static unsigned int blocksize_opt = 524288, chunksize_opt = 16384, blocks_opt = 2; /* accepted, unused */
+static uint64_t chunk_span = UINT64_C(1) << 28;
+static int host_primes = 0;
+static int help_opt = 0;
+
+static volatile sig_atomic_t stopping = 0;
+static int got_sigint = 0, got_sigterm = 0;
+
+/* ------------------------------------------------------------------ */
+/* framework services (main.h)                                          */
+/* ------------------------------------------------------------------ */
+FILE *bfopen(const char *filename, const char *mode) { return fopen(filename, mode); }
+void bmsg(const char *msg) { fprintf(stderr, "%s", msg); }
+char *bmprefix(void) { return ""; }
+void bexit(int status) { exit(status); }
+
+static uint64_t now_usec(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (uint64_t)ts.tv_sec * 1000000u + (uint64_t)ts.tv_nsec / 1000u;
+}
+
+static uint64_t cpu_usec(void) {
+  struct rusage ru;
+  getrusage(RUSAGE_SELF, &ru);
+  return (uint64_t)(ru.ru_utime.tv_sec + ru.ru_stime.tv_sec) * 1000000u +
+         (uint64_t)(ru.ru_utime.tv_usec + ru.ru_stime.tv_usec);
+}
+
+/* Integer option values with the suffix grammar of putil.c:86-168:
+ * K M G T P (x10^3..), k m g t p (x2^10..), eN (x10^N), bN (x2^N).
+ * Returns 0, -1 (cannot parse) or -2 (out of range). */
+int parse_uint64(uint64_t *result, const char *str, uint64_t lo, uint64_t hi) {
+  char *tail;
+  uint64_t num;
+  unsigned int e = 0;
+  errno = 0;
+  num = strtoull(str, &tail, 0);
+  if (errno != 0 || num > hi) return -2;
+  switch (*tail) {
+    case 'P': e += 3; /* fall through */
+    case 'T': e += 3; /* fall through */
+    case 'G': e += 3; /* fall through */
+    case 'M': e += 3; /* fall through */
+    case 'K':
+      e += 3;
+      if (tail[1] != '\0') return -1;
+      for (; e > 0; e -= 3) {
+        if (num > hi / 1000) return -2;
+        num *= 1000;
+      }
+      break;
+    case 'e':
+    case 'E':
+      e = (unsigned int)strtoul(tail + 1, &tail, 0);
+      if (errno != 0) return -2;
+      if (*tail != '\0') return -1;
+      while (e-- > 0) {
+        if (num > hi / 10) return -2;
+        num *= 10;
+      }
+      break;
+    case 'p': e += 10; /* fall through */
+    case 't': e += 10; /* fall through */
+    case 'g': e += 10; /* fall through */
+    case 'm': e += 10; /* fall through */
+    case 'k':
+      e += 10;
+      if (tail[1] != '\0') return -1;
+      if (num > (hi >> e)) return -2;
+      num <<= e;
+      break;
+    case 'b':
+    case 'B':
+      e = (unsigned int)strtoul(tail + 1, &tail, 0);
+      if (errno != 0) return -2;
+      if (*tail != '\0') return -1;
+      while (e-- > 0) {
+        if (num > (hi >> 1)) return -2;
+        num <<= 1;
+      }
+      break;
+    case '\0': break;
+    default: return -1;
+  }
+  if (num < lo) return -2;
+  *result = num;
+  return 0;
+}
+
+int parse_uint(unsigned int *result, const char *str, unsigned int lo, unsigned int hi) {
+  uint64_t v;
+  int st = parse_uint64(&v, str, lo, hi);
+  if (st == 0) *result = (unsigned int)v;
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* options                                                              */
+/* ------------------------------------------------------------------ */
+static const char *short_opts = "p:P:Q:B:C:c:r:t:z:qh" APP_SHORT_OPTS;
+static const struct option long_opts[] = {
+    {"pmin", required_argument, 0, 'p'},
+    {"pmax", required_argument, 0, 'P'},
+    {"qmax", required_argument, 0, 'Q'},
+    {"blocksize", required_argument, 0, 'B'},
+    {"chunksize", required_argument, 0, 'C'},
+    {"blocks", required_argument, 0, 256},
+    {"checkpoint", required_argument, 0, 'c'},
+    {"report", required_argument, 0, 'r'},
+    {"nthreads", required_argument, 0, 't'},
+    {"priority", required_argument, 0, 'z'},
+    {"help", no_argument, 0, 'h'},
+    {"quiet", no_argument, 0, 'q'},
+    {"host-primes", no_argument, 0, 258},
+    {"chunk", required_argument, 0, 259},
+    APP_LONG_OPTS{0, 0, 0, 0}};
+
+static void help(void) {
+  printf("-p --pmin=P0       Sieve start: 3 <= P0 <= p (default P0=3)\n");
+  printf("-P --pmax=P1       Sieve end: p < P1 <= 2^62 (default P1=P0+10^9)\n");
+  printf("-Q --qmax=Q1       Sieve only with odd primes q < Q1 <= 2^31\n");
+  printf("-c --checkpoint=N  Checkpoint every N seconds (default N=%u)\n", checkpoint_opt);
+  printf("-q --quiet         Don't print factors to screen\n");
+  printf("-r --report=N      Report status every N seconds (default N=%u)\n", report_opt);
+  printf("-t --nthreads=N    Start N threads on N GPUs (default N=1)\n");
+  printf("-z --priority=N    Set process priority to nice N or {idle,low,normal}\n");
+  printf("   --host-primes   Sieve primes on the host and pass them in batches\n");
+  printf("   --chunk=N       p-range handed to a worker at a time (default 2^28)\n");
+  printf("-h --help          Print this help\n");
+}
+
+static int parse_option(int opt, char *arg, const char *source) {
+  int status = 0;
+  switch (opt) {
+    case 'p':
+      free(pmin_str);
+      pmin_str = strdup(arg);
+      status = parse_uint64(&pmin, arg, PMIN_MIN, PMAX_MAX - 1);
+      break;
+    case 'P': status = parse_uint64(&pmax, arg, 4, PMAX_MAX); break;
+    case 'Q': status = parse_uint(&qmax_opt, arg, 3, QMAX_MAX); break;
+    case 'B': status = parse_uint(&blocksize_opt, arg, 1024, 1 << 27); break;
+    case 'C': status = parse_uint(&chunksize_opt, arg, 16, 1 << 19); break;
+    case 256: status = parse_uint(&blocks_opt, arg, 2, 4); break;
+    case 'c': status = parse_uint(&checkpoint_opt, arg, 0, UINT32_MAX); break;
+    case 'r': status = parse_uint(&report_opt, arg, 0, UINT32_MAX); break;
+    case 't': status = parse_uint((unsigned int *)&num_threads, arg, 1, MAX_THREADS); break;
+    case 'z':
+      if (strcmp(arg, "idle") == 0)
+        priority_opt = 19;
+      else if (strcmp(arg, "low") == 0)
+        priority_opt = 10;
+      else if (strcmp(arg, "normal") == 0)
+        priority_opt = 0;
+      else if (strcmp(arg, "none") == 0)
+        priority_opt = (unsigned int)-1;
+      else
+        status = parse_uint(&priority_opt, arg, 0, 19);
+      priority_opt++;
+      break;
+    case 'h': help_opt = 1; break;
+    case '?': status = -3; break;
+    case 'q': quiet_opt = 1; break;
+    case 258: host_primes = 1; break;
+    case 259: status = parse_uint64(&chunk_span, arg, 1 << 20, PMAX_MAX); break;
+    default: status = app_parse_option(opt, arg, source); break;
+  }
+  return status;
+}
+
+static void process_args(int argc, char *argv[]) {
+  int ind = -1, opt;
+  while ((opt = getopt_long(argc, argv, short_opts, long_opts, &ind)) != -1) {
+    switch (parse_option(opt, optarg, NULL)) {
+      case 0: ind = -1; break;
+      case -1:
+        if (ind == -1)
+          fprintf(stderr, "%s%s: invalid argument -%c %s\n", bmprefix(), argv[0], opt, optarg);
+        else
+          fprintf(stderr, "%s%s: invalid argument --%s %s\n", bmprefix(), argv[0], long_opts[ind].name, optarg);
+        exit(EXIT_FAILURE);
+      case -2:
+        if (ind == -1)
+          fprintf(stderr, "%s%s: out of range argument -%c %s\n", bmprefix(), argv[0], opt, optarg);
+        else
+          fprintf(stderr, "%s%s: out of range argument --%s %s\n", bmprefix(), argv[0], long_opts[ind].name,
+                  optarg);
+        exit(EXIT_FAILURE);
+      default: exit(EXIT_FAILURE);
+    }
+  }
+  while (optind < argc) {
+    if (parse_option(0, argv[optind], NULL) != 0) {
+      fprintf(stderr, "%s%s: invalid non-option argument %s\n", bmprefix(), argv[0], argv[optind]);
+      exit(EXIT_FAILURE);
+    }
+    optind++;
+  }
+  if (help_opt) {
+    help();
+    app_help();
+    exit(EXIT_SUCCESS);
+  }
+}
+
+/* "name=value" or "name value" lines, '#' comments (main.c:578-650). */
+static void read_config_file(const char *fn) {
+  const char *delim = "= \n\r\t\v";
+  char line[128];
+  FILE *f = bfopen(fn, "r");
+  if (!f) return;
+  while (fgets(line, sizeof line, f) != NULL) {
+    char *str = strtok(line, delim), *arg;
+    int ind;
+    if (str == NULL || str[0] == '#') continue;
+    arg = strtok(NULL, delim);
+    for (ind = 0; long_opts[ind].name; ind++)
+      if (strcmp(str, long_opts[ind].name) == 0) break;
+    if (long_opts[ind].name == NULL) {
+      fprintf(stderr, "%s%s: unrecognised option `%s'\n", bmprefix(), fn, str);
+      exit(EXIT_FAILURE);
+    }
+    if (long_opts[ind].has_arg == no_argument && arg != NULL) {
+      fprintf(stderr, "%s%s: option `%s' doesn't allow an argument\n", bmprefix(), fn, str);
+      exit(EXIT_FAILURE);
+    } else if (long_opts[ind].has_arg == required_argument && arg == NULL) {
+      fprintf(stderr, "%s%s: option `%s' requires an argument\n", bmprefix(), fn, str);
+      exit(EXIT_FAILURE);
+    }
+    switch (parse_option(long_opts[ind].val, arg, fn)) {
+      case 0: break;
+      case -1: fprintf(stderr, "%s%s: invalid argument %s %s\n", bmprefix(), fn, str, arg); exit(EXIT_FAILURE);
+      case -2:
+        fprintf(stderr, "%s%s: out of range argument %s %s\n", bmprefix(), fn, str, arg);
+        exit(EXIT_FAILURE);
+      default: fprintf(stderr, "%s%s: weird argument %s %s\n", bmprefix(), fn, str, arg); exit(EXIT_FAILURE);
+    }
+  }
+  fclose(f);
+}
+
+/* ------------------------------------------------------------------ */
+/* work distribution and checkpoints                                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  uint64_t p0, p1, count, sum;
+  int done;
+} chunk_t;
+
+static pthread_mutex_t mtx = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t cond = PTHREAD_COND_INITIALIZER;
+static chunk_t *chunks;
+static size_t nchunks, next_chunk_ix, frontier_ix;
+static uint64_t cand_count, cand_sum; /* primes below the frontier (incl. resumed) */
+static int checkpoint_request, idle_workers, live_workers, unusable_workers, failed;
+
+/* Advance the frontier over completed chunks; caller holds mtx. */
+static void advance_frontier(void) {
+  while (frontier_ix < nchunks && chunks[frontier_ix].done) {
+    cand_count += chunks[frontier_ix].count;
+    cand_sum += chunks[frontier_ix].sum;
+    frontier_ix++;
+  }
+}
+
+static uint64_t progress_p(void) { return frontier_ix < nchunks ? chunks[frontier_ix].p0 : pmax; }
+
+static void write_checkpoint(uint64_t p) {
+  FILE *fout = fopen(checkpoint_filename, "w");
+  if (!fout) return;
+  fprintf(fout, "pmin=%" PRIu64 ",p=%" PRIu64 ",count=%" PRIu64 ",sum=0x%016" PRIx64 ",checksum=0x%016" PRIx64 "\n",
+          pmin, p, cand_count, cand_sum, pmin + p + cand_count + cand_sum);
+  app_write_checkpoint(fout);
+  fclose(fout);
+}
+
+/* main.c:332-372: a checkpoint is used only when its pmin matches, pmin <
+ * p < pmax, the application accepts its second line and the checksum holds. */
+static uint64_t read_checkpoint(void) {
+  uint64_t p0, p, count, sum, checksum;
+  const char *cpf = checkpoint_filename;
+  int valid = 0;
+  FILE *fin = fopen(cpf, "r");
+  if (!fin) {
+    if ((fin = fopen(OLD_CHECKPOINT_FILENAME, "r")) == NULL) return pmin;
+    cpf = OLD_CHECKPOINT_FILENAME;
+  }
+  if (fscanf(fin, "pmin=%" SCNu64 ",p=%" SCNu64 ",count=%" SCNu64 ",sum=0x%" SCNx64 ",checksum=0x%" SCNx64 "\n", &p0,
+             &p, &count, &sum, &checksum) == 5)
+    if (p0 == pmin && p > pmin && p < pmax) valid = app_read_checkpoint(fin);
+  fclose(fin);
+  if (valid && p0 + p + count + sum == checksum) {
+    fprintf(stderr, "%sResuming from checkpoint p=%" PRIu64 " in %s\n", bmprefix(), p, cpf);
+    cand_count = count;
+    cand_sum = sum;
+    return p;
+  }
+  fprintf(stderr, "%sIgnoring invalid checkpoint in %s\n", bmprefix(), cpf);
+  return pmin;
+}
+
+/* Next chunk for a worker, or NULL when done/stopping; honours checkpoint
+ * requests by parking the worker until the checkpoint is written. */
+static chunk_t *take_chunk(void) {
+  chunk_t *c = NULL;
+  pthread_mutex_lock(&mtx);
+  while (checkpoint_request && !stopping) {
+    idle_workers++;
+    pthread_cond_broadcast(&cond);
+    while (checkpoint_request && !stopping) pthread_cond_wait(&cond, &mtx);
+    idle_workers--;
+  }
+  if (!stopping && !failed && next_chunk_ix < nchunks) c = &chunks[next_chunk_ix++];
+  pthread_mutex_unlock(&mtx);
+  return c;
+}
+
+static void finish_chunk(chunk_t *c, uint64_t count, uint64_t sum) {
+  pthread_mutex_lock(&mtx);
+  c->count = count;
+  c->sum = sum;
+  c->done = 1;
+  advance_frontier();
+  pthread_cond_broadcast(&cond);
+  pthread_mutex_unlock(&mtx);
+}
+
+/* odd primes <= lim (host sieve for --host-primes) */
+static uint32_t *host_sieve_primes(uint32_t lim, uint32_t *n) {
+  uint8_t *comp = calloc((size_t)lim / 2 + 1, 1);
+  uint32_t *out, cnt = 0;
+  for (uint64_t i = 3; i * i <= lim; i += 2)
+    if (!comp[i / 2])
+      for (uint64_t m = i * i; m <= lim; m += 2 * i) comp[m / 2] = 1;
+  for (uint64_t i = 3; i <= lim; i += 2) cnt += !comp[i / 2];
+  out = malloc(sizeof(uint32_t) * (cnt + 1));
+  cnt = 0;
+  for (uint64_t i = 3; i <= lim; i += 2)
+    if (!comp[i / 2]) out[cnt++] = (uint32_t)i;
+  free(comp);
+  *n = cnt;
+  return out;
+}
+
+static uint32_t *hq;
+static uint32_t nhq;
+
+static void *thread_fun(void *arg) {
+  const int th = (int)(long)arg;
+  unsigned int cthread_count = app_thread_init(th);
+  uint64_t *P = NULL, *lastP = NULL;
+  uint8_t *seg = NULL;
+  const uint64_t SEG = UINT64_C(1) << 20;
+  fprintf(stderr, "%sThread %d starting\n", bmprefix(), th);
+  if (cthread_count == 0) {
+    pthread_mutex_lock(&mtx);
+    unusable_workers++;
+    live_workers--;
+    pthread_cond_broadcast(&cond);
+    pthread_mutex_unlock(&mtx);
+    return NULL;
+  }
+  if (host_primes) {
+    P = malloc(sizeof(uint64_t) * cthread_count);
+    lastP = malloc(sizeof(uint64_t) * cthread_count);
+    seg = malloc(SEG);
+  }
+  for (chunk_t *c; (c = take_chunk()) != NULL;) {
+    uint64_t count = 0, sum = 0;
+    if (!host_primes) {
+      int rc = app_thread_range(th, c->p0, c->p1, &count, &sum);
+      if (rc) {
+        fprintf(stderr, "%sThread %d: device error %d\n", bmprefix(), th, rc);
+        pthread_mutex_lock(&mtx);
+        failed = 1;
+        pthread_cond_broadcast(&cond);
+        pthread_mutex_unlock(&mtx);
+        break;
+      }
+    } else {
+      /* main.c:736-832: odd candidates of the chunk, batches of cthread_count */
+      unsigned int plen = 0;
+      for (uint64_t base = c->p0 | 1; base < c->p1; base += 2 * SEG) {
+        uint64_t cnt = (c->p1 - base + 1) / 2;
+        if (cnt > SEG) cnt = SEG;
+        memset(seg, 0, cnt);
+        for (uint32_t i = 0; i < nhq; i++) {
+          uint64_t q = hq[i], first, r;
+          if (q * q >= base + 2 * cnt) break;
+          r = base % q;
+          first = r ? base + (q - r) : base;
+          if (!(first & 1)) first += q;
+          if (first < q * q) first = q * q;
+          for (uint64_t m = first; m < base + 2 * cnt; m += 2 * q) seg[(m - base) / 2] = 1;
+        }
+        for (uint64_t j = 0; j < cnt; j++)
+          if (!seg[j]) {
+            const uint64_t p = base + 2 * j;
+            count++;
+            sum += p;
+            P[plen++] = p;
+            if (plen == cthread_count) {
+              app_thread_fun(th, P, lastP, cthread_count);
+              plen = 0;
+            }
+          }
+      }
+      app_thread_fun1(th, P, lastP, cthread_count, plen);
+    }
+    finish_chunk(c, count, sum);
+  }
+  app_thread_fini(th);
+  free(P);
+  free(lastP);
+  free(seg);
+  pthread_mutex_lock(&mtx);
+  live_workers--;
+  pthread_cond_broadcast(&cond);
+  pthread_mutex_unlock(&mtx);
+  return NULL;
+}
+
+static void handle_signal(int signum) {
+  if (signum == SIGINT) got_sigint = 1;
+  if (signum == SIGTERM) got_sigterm = 1;
+  stopping = 1;
+}
+
+static void report_status(uint64_t t_now, uint64_t t_start, uint64_t progress, uint64_t last_t, uint64_t last_p,
+                          uint64_t cpu_now, uint64_t cpu_last) {
+  double rate = (double)(progress - last_p) / (double)(t_now - last_t + 1); /* p per usec = M p/sec */
+  const char *unit = "M";
+  int prec;
+  if (rate < 1.0) rate *= 1000.0, unit = "K";
+  if (rate < 1.0) rate *= 1000.0, unit = "";
+  prec = rate < 10.0 ? 3 : rate < 100.0 ? 2 : rate < 1000.0 ? 1 : 0;
+  (void)t_start;
+  printf("p=%" PRIu64 ", %.*f%s p/sec, %.2f CPU cores, %.1f%% done.  ", progress, prec, rate, unit,
+         (double)(cpu_now - cpu_last) / (double)(t_now - last_t + 1), 100.0 * (double)(progress - pmin) / (double)(pmax - pmin));
+  if (quiet_opt) {
+    putchar('\r');
+    fflush(stdout);
+  } else
+    putchar('\n');
+}
+
+int main(int argc, char *argv[]) {
+  const uint64_t program_start = now_usec();
+  uint64_t sieve_start, cpu_start, pstop, last_ckpt, last_rep, last_rep_p, last_rep_cpu;
+  pthread_t tid[MAX_THREADS];
+  int th, process_ret = EXIT_SUCCESS;
+
+  setvbuf(stdout, NULL, _IOLBF, 0);
+  app_banner();
+  read_config_file(CONFIG_FILENAME);
+  process_args(argc, argv);
+
+  if (pmin < PMIN_MIN) pmin = PMIN_MIN;
+  if (pmax > PMAX_MAX) pmax = PMAX_MAX;
+  if (pmin >= pmax) {
+    if (pmax == 0 && pmin < PMAX_MAX - 1000000000) {
+      fprintf(stderr, "%spmax not specified, using default pmax = pmin + 1e9\n", bmprefix());
+      pmax = pmin + 1000000000;
+    } else {
+      fprintf(stderr, "%sOption out of range: pmax must be greater than pmin\n", bmprefix());
+      exit(EXIT_FAILURE);
+    }
+  }
+  if (asprintf(&checkpoint_filename, CHECKPOINT_FILENAME, pmin_str ? pmin_str : "") < 0) exit(EXIT_FAILURE);
+  if (priority_opt) setpriority(PRIO_PROCESS, 0, (int)priority_opt - 1);
+  if (qmax_opt && (uint64_t)qmax_opt * qmax_opt > pmax) qmax_opt = 0; /* default: all <= sqrt(pmax) */
+
+  app_init();
+  fprintf(stderr, "%sSieve started: %" PRIu64 " <= p < %" PRIu64 "\n", bmprefix(), pmin, pmax);
+
+  pstart = read_checkpoint() | 1;
+
+  if (host_primes) {
+    uint64_t lim = (uint64_t)sqrt((double)(pmax - 1));
+    while (lim * lim > pmax - 1) lim--;
+    while ((lim + 1) * (lim + 1) <= pmax - 1) lim++;
+    if (qmax_opt && qmax_opt < lim) lim = qmax_opt < 61 ? 61 : qmax_opt;
+    hq = host_sieve_primes((uint32_t)lim, &nhq);
+  }
+
+  /* chunks [pstart, pmax) */
+  nchunks = (size_t)((pmax - pstart + chunk_span - 1) / chunk_span);
+  if (pstart >= pmax) nchunks = 0;
+  chunks = calloc(nchunks + 1, sizeof(chunk_t));
+  for (size_t i = 0; i < nchunks; i++) {
+    chunks[i].p0 = pstart + i * chunk_span;
+    chunks[i].p1 = (i + 1 == nchunks) ? pmax : chunks[i].p0 + chunk_span;
+  }
+
+  signal(SIGINT, handle_signal);
+  signal(SIGTERM, handle_signal);
+  signal(SIGHUP, handle_signal);
+
+  sieve_start = now_usec();
+  cpu_start = cpu_usec();
+  last_ckpt = last_rep = sieve_start;
+  last_rep_p = pstart;
+  last_rep_cpu = cpu_start;
+  live_workers = num_threads;
+  for (th = 0; th < num_threads; th++) pthread_create(&tid[th], NULL, thread_fun, (void *)(long)th);
+
+  pthread_mutex_lock(&mtx);
+  while (live_workers > 0) {
+    struct timespec ts;
+    uint64_t t = now_usec(), wake;
+    if (report_opt && t >= last_rep + (uint64_t)report_opt * 1000000u) {
+      uint64_t cpu = cpu_usec();
+      report_status(t, sieve_start, progress_p(), last_rep, last_rep_p, cpu, last_rep_cpu);
+      last_rep = t;
+      last_rep_p = progress_p();
+      last_rep_cpu = cpu;
+    }
+    if (checkpoint_opt && !stopping && t >= last_ckpt + (uint64_t)checkpoint_opt * 1000000u) {
+      /* park every worker between chunks (main.c:1063-1098) */
+      checkpoint_request = 1;
+      while (idle_workers < live_workers && !stopping) pthread_cond_wait(&cond, &mtx);
+      write_checkpoint(progress_p());
+      checkpoint_request = 0;
+      pthread_cond_broadcast(&cond);
+      last_ckpt = now_usec();
+    }
+    wake = t + 200000;
+    clock_gettime(CLOCK_REALTIME, &ts);
+    ts.tv_nsec += 200000000;
+    if (ts.tv_nsec >= 1000000000) ts.tv_sec++, ts.tv_nsec -= 1000000000;
+    (void)wake;
+    pthread_cond_timedwait(&cond, &mtx, &ts);
+    if (stopping) pthread_cond_broadcast(&cond);
+  }
+  pthread_mutex_unlock(&mtx);
+  for (th = 0; th < num_threads; th++) pthread_join(tid[th], NULL);
+  if (failed || unusable_workers == num_threads) process_ret = EXIT_FAILURE;
+
+  /* a failed run leaves the last checkpoint as it was (main.c:1196-1204) */
+  pstop = progress_p();
+  if (pstop > pmax) pstop = pmax;
+  if (process_ret == EXIT_SUCCESS)
+    write_checkpoint(pstop);
+  else
+    pstop = pstart;
+  if (pstop >= pmax) {
+    fprintf(stderr, "\n%sSieve complete: %" PRIu64 " <= p < %" PRIu64 "\n", bmprefix(), pmin, pmax);
+    remove(checkpoint_filename);
+  } else
+    fprintf(stderr, "\n%sSieve incomplete: %" PRIu64 " <= p < %" PRIu64 "\n", bmprefix(), pmin, pstop);
+  app_fini(pstop);
+  if (process_ret == EXIT_SUCCESS)
+    fprintf(stderr, "%scount=%" PRIu64 ",sum=0x%016" PRIx64 "\n", bmprefix(), cand_count, cand_sum);
+  {
+    const uint64_t stop = now_usec(), cpu = cpu_usec();
+    fprintf(stderr, "%sElapsed time: %.2f sec. (%.2f init + %.2f sieve) at %.0f p/sec.\n", bmprefix(),
+            (stop - program_start) / 1e6, (sieve_start - program_start) / 1e6, (stop - sieve_start) / 1e6,
+            (double)(pstop - pstart) / (double)(stop - sieve_start + 1) * 1e6);
+    fprintf(stderr, "%sProcessor time: %.2f sec. at %.0f p/sec.\n", bmprefix(), cpu / 1e6,
+            (double)(pstop - pstart) / (double)(cpu - cpu_start + 1) * 1e6);
+  }
+  if (got_sigint) {
+    signal(SIGINT, SIG_DFL);
+    raise(SIGINT);
+  }
+  if (got_sigterm) {
+    signal(SIGTERM, SIG_DFL);
+    raise(SIGTERM);
+  }
+  return process_ret;
+}

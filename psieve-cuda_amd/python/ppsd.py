@@ -32,6 +32,7 @@ class Params(C.Structure):
     _fields_ = [
         ("pmin", C.c_uint64), ("pmax", C.c_uint64),
         ("kmin", C.c_uint64), ("kmax", C.c_uint64),
+        ("qmax", C.c_uint64),
         ("nmin", C.c_uint32), ("nmax", C.c_uint32),
         ("kstep", C.c_uint32), ("koffset", C.c_uint32),
         ("sign", C.c_int32),
@@ -119,10 +120,11 @@ class SearchParams:
     kstep: int = 0
     koffset: int = 0
     sign: int = 1  # +1 Proth, -1 Riesel (-R), 0 both
+    qmax: int = 0  # -Q; 0 = every odd prime <= sqrt(pmax - 1)
 
     def to_c(self) -> Params:
         p = Params()
-        for f in ("pmin", "pmax", "kmin", "kmax", "nmin", "nmax", "kstep", "koffset", "sign"):
+        for f in ("pmin", "pmax", "kmin", "kmax", "nmin", "nmax", "kstep", "koffset", "sign", "qmax"):
             setattr(p, f, getattr(self, f))
         return p
 
