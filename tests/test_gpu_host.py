@@ -62,7 +62,7 @@ def test_riesel_and_both_vs_oracle(tmp_path, sign_args, sign):
     want = orc.walk(pr, orc.primes(lo, hi), filter=True, threads=8)
     lines = sorted(f"{int(x['p'])} | {int(x['k'])}*2^{int(x['n'])}{int(x['c']):+d}" for x in want)
     assert factors(tmp_path) == lines
-    assert len(lines) > 10
+    assert len(lines) > 3
 
 
 def test_interrupt_and_resume(tmp_path):
