@@ -26,16 +26,16 @@ METRIC = "primes/sec sieved (k≤1e4, n≤6e6) batched vs plain; % of INT32 VALU
 
 WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000, sign=1)
 
-# INT32 VALU peak of one MI355X: 256 CUs x 64 lanes x 2.4 GHz lane-ops/s
-# (published 157.3 TFLOP/s FP32 vector = that x 2 for FMA x 2 for packed FP32,
-# which integer ops do not have).
-PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
-# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32, with a
-# MUL32 costing R_MUL full-rate issue slots (measured on gfx950: v_mul_lo_u32,
-# v_mul_hi_u32 and v_mad_u64_u32 issue at a quarter of v_add_u32's rate).
+# INT32 VALU peak of one MI355X: 256 CUs x 128 lanes x 2.4 GHz = 78.6 TOP/s,
+# i.e. the published 157.3 TFLOP/s FP32 vector peak / 2 flops per FMA.  Simple
+# 32-bit ops (v_add_u32, v_xor_b32, v_mov_b32) issue at that rate on gfx950.
+PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32 lane-ops,
+# a MUL32 weighted by R_MUL ALU32 slots: multiplies (v_mul_lo_u32,
+# v_mul_hi_u32, v_mad_u64_u32) issue at half the simple-op rate (measured).
 MUL32_PER_STEP = 4
 ALU32_PER_STEP = 11
-R_MUL = 4.0
+R_MUL = 2.0
 
 
 def parse():

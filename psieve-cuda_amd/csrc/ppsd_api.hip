@@ -96,6 +96,7 @@ struct ppsd_ctx {
   Pending pending = Pending::none;
   uint32_t n_primes_host = 0;  // prime count when known on the host
   bool sieved = false;
+  uint64_t range_lo = 0, range_nbits = 0;  // candidates of a range submission
   std::vector<Hit> result;
   bool result_raw = false;
   uint64_t last_primes = 0, last_sum = 0;
@@ -410,6 +411,38 @@ int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
   return 0;
 }
 
+// Odd candidates from lo whose expected prime count (2/ln x of them),
+// plus six standard deviations, fits the batch.  A rarer excess is handled
+// in finish() by sieving the chunk again in halves.
+static uint64_t range_fit(const ppsd_ctx *c, uint64_t lo) {
+  const double dens = 2.0 / log((double)lo);
+  const double room = (double)c->batch - 32.0;
+  if (room <= 1.0) return 1;
+  const double r = (-6.0 + sqrt(36.0 + 4.0 * room)) / 2.0;  // mu + 6 sqrt(mu) = room
+  uint64_t fit = (uint64_t)(r * r / dens);
+  fit = std::min<uint64_t>(fit, c->nbits_max);
+  return fit ? fit : 1;
+}
+
+// Sieve odd candidates [lo, lo + 2 nbits) into d_P (count in d_count).
+static int enqueue_sieve(ppsd_ctx *c, uint64_t lo, uint64_t nbits) {
+  HIP_TRY(launch_zero_u32(c->d_count, 1, c->stream));
+  if (!nbits) return 0;
+  SieveArgs sa{};
+  sa.lo = lo;
+  sa.nbits = nbits;
+  sa.q = c->d_sp;
+  sa.nq_small = c->nsp_small;
+  sa.nq = c->nsp;
+  sa.bitmap = c->d_bitmap;
+  HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  HIP_TRY(launch_sieve(sa, c->stream));
+  HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->batch, c->d_count, c->d_counters + 2,
+                           c->stream));
+  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  return 0;
+}
+
 int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
   if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "previous submission not collected");
@@ -419,34 +452,16 @@ int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
   HIP_TRY(hipSetDevice(c->device));
   const uint64_t lo = p0 | 1;
   uint64_t nbits = p1 > lo ? (p1 - lo + 1) / 2 : 0;
-  // keep the expected prime count (2 per ln x odd candidates) 10% under the
-  // batch; segment-aligned so chunks tile the range
-  const double dens = 2.0 / log((double)lo);
-  uint64_t fit = (uint64_t)(0.9 * c->batch / dens);
-  fit = std::min<uint64_t>(fit, c->nbits_max);
-  fit = fit / kSegBits * kSegBits;
-  if (fit == 0) fit = kSegBits;
-  if (nbits > fit) nbits = fit;
+  nbits = std::min<uint64_t>(nbits, range_fit(c, lo));
   const uint64_t stop = nbits ? std::min<uint64_t>(lo + 2 * nbits, p1) : p1;
   HIP_TRY(launch_zero_u64(c->d_counters, 4, c->stream));
-  HIP_TRY(launch_zero_u32(c->d_count, 1, c->stream));
-  if (nbits) {
-    SieveArgs sa{};
-    sa.lo = lo;
-    sa.nbits = nbits;
-    sa.q = c->d_sp;
-    sa.nq_small = c->nsp_small;
-    sa.nq = c->nsp;
-    sa.bitmap = c->d_bitmap;
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    HIP_TRY(launch_sieve(sa, c->stream));
-    HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->batch, c->d_count,
-                             c->d_counters + 2, c->stream));
-    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  }
-  int rc = enqueue_walk(c, 0, 0, c->d_count, c->batch);
+  int rc = enqueue_sieve(c, lo, nbits);
+  if (rc) return rc;
+  rc = enqueue_walk(c, 0, 0, c->d_count, c->batch);
   if (rc) return rc;
   c->pending = Pending::range;
+  c->range_lo = lo;
+  c->range_nbits = nbits;
   c->sieved = nbits != 0;
   c->have_result = false;
   if (p_next) *p_next = stop;
@@ -457,6 +472,76 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
   return ms;
+}
+
+// Walk d_P[first..first+n) synchronously and append its hits to the result,
+// halving the piece whenever the raw ring overflows.
+static int walk_pieces(ppsd_ctx *c, bool raw, uint32_t first, uint32_t n, uint64_t &raw_total, uint64_t &kept_total) {
+  std::vector<std::pair<uint32_t, uint32_t>> todo{{first, n}};
+  while (!todo.empty()) {
+    auto [f, k] = todo.back();
+    todo.pop_back();
+    if (k == 0) continue;
+    unsigned long long cnt[2];
+    HIP_TRY(launch_zero_u64(c->d_counters, 2, c->stream));
+    int rc = enqueue_walk(c, f, k, nullptr, k);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
+    c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
+    if (cnt[0] > c->hit_cap) {
+      if (k == 1)
+        return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "one prime has more than %llu hits",
+                       (unsigned long long)c->hit_cap);
+      c->st.retries++;
+      todo.push_back({f + k / 2, k - k / 2});
+      todo.push_back({f, k / 2});
+      continue;
+    }
+    const uint64_t take = raw ? cnt[0] : cnt[1];
+    const size_t old = c->result.size();
+    c->result.resize(old + take);
+    if (take)
+      HIP_TRY(hipMemcpy(c->result.data() + old, raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
+    raw_total += cnt[0];
+    kept_total += cnt[1];
+  }
+  return 0;
+}
+
+// A range submission whose chunk held more primes than the batch: sieve and
+// walk [lo, lo + 2 nbits) again in halves until each piece fits.
+static int rewalk_range(ppsd_ctx *c, bool raw, uint64_t lo, uint64_t nbits, uint64_t &raw_total,
+                        uint64_t &kept_total) {
+  std::vector<std::pair<uint64_t, uint64_t>> todo{{lo, nbits}};
+  uint64_t primes = 0, sum = 0;
+  while (!todo.empty()) {
+    auto [l, nb] = todo.back();
+    todo.pop_back();
+    if (nb == 0) continue;
+    unsigned long long s2 = 0;
+    uint32_t n = 0;
+    HIP_TRY(launch_zero_u64(c->d_counters, 4, c->stream));
+    int rc = enqueue_sieve(c, l, nb);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(&n, c->d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&s2, c->d_counters + 2, sizeof s2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);
+    if (n > c->batch) {
+      todo.push_back({l + 2 * (nb / 2), nb - nb / 2});
+      todo.push_back({l, nb / 2});
+      continue;
+    }
+    rc = walk_pieces(c, raw, 0, n, raw_total, kept_total);
+    if (rc) return rc;
+    primes += n;
+    sum += s2;
+  }
+  c->last_primes = primes;
+  c->last_sum = sum;
+  return 0;
 }
 
 // Wait for the submission; gather kept (and raw) hits, re-walking in halves
@@ -475,57 +560,42 @@ static int finish(ppsd_ctx *c, bool raw) {
   c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
   c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
   if (c->sieved) c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);
+  c->result.clear();
+  c->result_raw = raw;
+  uint64_t raw_total = 0, kept_total = 0;
   if (c->pending == Pending::range) {
     c->last_primes = nprimes;
     c->last_sum = cnt[2];
     if (nprimes > c->batch) {
+      // more primes than the batch holds: sieve and walk the chunk again in
+      // pieces small enough to fit, synchronously
+      c->st.retries++;
+      int rc = rewalk_range(c, raw, c->range_lo, c->range_nbits, raw_total, kept_total);
       c->pending = Pending::none;
-      return set_err(PPSD_ERR_FAILURE, "device prime count %u exceeded the batch %u", nprimes, c->batch);
+      if (rc) return rc;
+      c->st.primes += c->last_primes;
+      c->st.prime_sum += c->last_sum;
+      c->st.window_steps += c->last_primes * windows(c->prm) * (c->prm.sign == 0 ? 2 : 1);
+      c->st.raw_hits += raw_total;
+      c->st.factors += kept_total;
+      c->have_result = true;
+      return 0;
     }
   }
-  c->result.clear();
-  c->result_raw = raw;
-  uint64_t raw_total = 0, kept_total = 0;
-  // pieces to (re)walk: (first, n)
-  std::vector<std::pair<uint32_t, uint32_t>> todo;
   if (cnt[0] <= c->hit_cap) {
     const uint64_t take = raw ? cnt[0] : cnt[1];
-    const size_t old = c->result.size();
-    c->result.resize(old + take);
+    c->result.resize(take);
     if (take)
-      HIP_TRY(hipMemcpy(c->result.data() + old, raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
-    raw_total += cnt[0];
-    kept_total += cnt[1];
+      HIP_TRY(hipMemcpy(c->result.data(), raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
+    raw_total = cnt[0];
+    kept_total = cnt[1];
   } else {
     c->st.retries++;
-    todo.push_back({0, nprimes / 2});
-    todo.push_back({nprimes / 2, nprimes - nprimes / 2});
-  }
-  while (!todo.empty()) {
-    auto [first, n] = todo.back();
-    todo.pop_back();
-    if (n == 0) continue;
-    HIP_TRY(launch_zero_u64(c->d_counters, 2, c->stream));
-    int rc = enqueue_walk(c, first, n, nullptr, n);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
-    c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
-    if (cnt[0] > c->hit_cap) {
-      if (n == 1) return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "one prime has more than %llu hits", (unsigned long long)c->hit_cap);
-      c->st.retries++;
-      todo.push_back({first, n / 2});
-      todo.push_back({first + n / 2, n - n / 2});
-      continue;
+    int rc = walk_pieces(c, raw, 0, nprimes, raw_total, kept_total);
+    if (rc) {
+      c->pending = Pending::none;
+      return rc;
     }
-    const uint64_t take = raw ? cnt[0] : cnt[1];
-    const size_t old = c->result.size();
-    c->result.resize(old + take);
-    if (take)
-      HIP_TRY(hipMemcpy(c->result.data() + old, raw ? c->d_hits : c->d_kept, take * sizeof(Hit), hipMemcpyDeviceToHost));
-    raw_total += cnt[0];
-    kept_total += cnt[1];
   }
   const uint64_t signs = c->prm.sign == 0 ? 2 : 1;
   c->st.primes += nprimes;

@@ -62,25 +62,28 @@ __device__ __forceinline__ uint64_t inv2pow(uint64_t p, uint64_t pinv, uint32_t 
   return mh ? p - mh : 0;
 }
 
-// One window: K <- K * 2^-s (mod p) for 1 <= s <= 32.
-//   j = K * p^-1 mod 2^s (only K's low word matters), so K - j*p == 0 mod 2^s
-//   and (K - j*p) / 2^s = (K >> s) - ((j*p) >> s), the second term computed as
-//   (m*p) >> 32 = m*p_hi + mulhi(m, p_lo) with m = j << (32-s).  Folding the
-//   shift into the inverse, m = K_lo * (pinv32 << (32-s)) mod 2^32.
-// The state is kept redundant: K' = (K >> s) + (p - jp) lies in
-// [1, p + p/(2^s - 1)) whenever K does, so no per-step reduction is needed;
-// K >= p (rare: needs K >> s >= jp) is caught by the pre-test and reduced
-// only where a candidate is examined.
+// One window: K <- K * 2^-s (mod p) for 1 <= s <= 32, as the exact division
+// K' = (K + j*p) / 2^s with j = K * (-p^-1) mod 2^s (so K + j*p == 0 mod 2^s).
+// Splitting off the low s bits, K' = (K >> s) + ceil(j*p / 2^s), and with
+// m = j * 2^(32-s):  ceil(j*p / 2^s) = m*p_hi + ceil(m*p_lo / 2^32), where
+// ceil(m*p_lo / 2^32) is the high word of m*p_lo + (2^32 - 1).  Folding the
+// 2^(32-s) into the inverse, m = K_lo * ps_sh mod 2^32 with
+// ps_sh = (-p^-1 mod 2^32) << (32-s).
+// Every term is non-negative, so there is no borrow to repair, and the state
+// stays in [1, 2p) (K < 2p gives K' < p + p/2^s): no per-step reduction.  A
+// state >= p is caught by the walk's pre-test and reduced only where a
+// candidate is examined.
 template <bool S32>
-__device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t pinv_sh, uint32_t s) {
-  const uint32_t m = lo32(K) * pinv_sh;
-  const uint32_t h = __umulhi(m, lo32(p));
-  const uint64_t jp = (uint64_t)m * hi32(p) + h;
+__device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s) {
+  const uint32_t m = lo32(K) * ps_sh;
+  const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
   const uint64_t ks = S32 ? (uint64_t)hi32(K) : (K >> s);
-  // (ks - jp) wraps when negative; adding p brings it back into range
-  uint64_t d;
-  __builtin_sub_overflow(ks, jp, &d);
-  return d + p;
+  return (uint64_t)m * hi32(p) + (ks + hc);
+}
+
+// The multiplier of step() for prime p and window s.
+__device__ __forceinline__ uint32_t step_multiplier(uint64_t pinv, uint32_t s) {
+  return (0u - lo32(pinv)) << (32u - s);
 }
 
 // Find-first-set-bit of the low word, as v_ffbl_b32: 0xffffffff when the word

@@ -60,30 +60,40 @@ __device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
   return (int)unreduced | (int)pretest(K, kmax);
 }
 
+// Per-lane constants of the R primes a lane walks, and a snapshot of their
+// states; passed by value so the rare re-walk call needs no scratch memory.
+template <int R>
+struct LanePrimes {
+  uint64_t p[R];
+  uint32_t ps[R];
+  bool valid[R];
+};
+template <int R>
+struct LaneStates {
+  uint64_t K[R];
+};
+
 // Re-walk `cnt` windows from state K (window start n) with exact tests and
 // emission.  Reached only when a pre-test fired somewhere in the chunk.
 template <int SIGN, int R, bool S32>
-__device__ __noinline__ void rescan(const uint64_t (&p)[R], const uint32_t (&pinv_sh)[R], const bool (&valid)[R],
-                                    const uint64_t (&K0)[R], uint32_t n, uint32_t cnt, const WalkArgs &a) {
-  uint64_t K[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) K[r] = K0[r];
+__device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
+                                    const WalkArgs &a) {
   for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint64_t Kc = K[r] >= p[r] ? K[r] - p[r] : K[r];
+      const uint64_t Kc = S.K[r] >= L.p[r] ? S.K[r] - L.p[r] : S.K[r];
       uint64_t k = 0;
       uint32_t nn = 0;
       if (SIGN == 0) {
-        bool h = valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
-        emit(h, p[r], k, nn, -1, a);
-        h = valid[r] && exact(p[r] - Kc, n, a, k, nn);
-        emit(h, p[r], k, nn, +1, a);
+        bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
+        emit(h, L.p[r], k, nn, -1, a);
+        h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
+        emit(h, L.p[r], k, nn, +1, a);
       } else {
-        const bool h = valid[r] && exact(Kc, n, a, k, nn);
-        emit(h, p[r], k, nn, SIGN, a);
+        const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
+        emit(h, L.p[r], k, nn, SIGN, a);
       }
-      K[r] = step<S32>(K[r], p[r], pinv_sh[r], a.s);
+      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
     }
   }
 }
@@ -98,44 +108,41 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   const uint32_t count = cnt < a.cap ? cnt : a.cap;
   const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * R;
   if (first >= count) return;
-  uint64_t p[R], K[R];
-  uint32_t pinv_sh[R];
-  bool valid[R];
+  LanePrimes<R> L;
+  LaneStates<R> S;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    valid[r] = first + r < count;
-    p[r] = a.P[valid[r] ? first + r : count - 1];
-    const uint64_t pinv = inv64(p[r]);
-    pinv_sh[r] = lo32(pinv) << (32u - a.s);
-    const uint64_t x = inv2pow(p[r], pinv, a.nmin);
-    K[r] = (SIGN == 1) ? p[r] - x : x;
+    L.valid[r] = first + r < count;
+    L.p[r] = a.P[L.valid[r] ? first + r : count - 1];
+    const uint64_t pinv = inv64(L.p[r]);
+    L.ps[r] = step_multiplier(pinv, a.s);
+    const uint64_t x = inv2pow(L.p[r], pinv, a.nmin);
+    S.K[r] = (SIGN == 1) ? L.p[r] - x : x;
   }
   const uint32_t s = a.s;
   uint32_t n = a.nmin;
   uint32_t w = 0;
   for (; w + U <= a.nwin; w += U) {
-    uint64_t Ks[R];
+    const LaneStates<R> snap = S;
     bool any = false;
-#pragma unroll
-    for (int r = 0; r < R; ++r) Ks[r] = K[r];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        any |= pre<SIGN>(K[r], p[r], a.kmax);
-        K[r] = step<S32>(K[r], p[r], pinv_sh[r], s);
+        any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+        S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
       }
     }
-    if (any) rescan<SIGN, R, S32>(p, pinv_sh, valid, Ks, n, U, a);
+    if (any) rescan<SIGN, R, S32>(L, snap, n, U, a);
     n += U * s;
   }
   for (; w < a.nwin; ++w, n += s) {
     bool any = false;
 #pragma unroll
-    for (int r = 0; r < R; ++r) any |= pre<SIGN>(K[r], p[r], a.kmax);
-    if (any) rescan<SIGN, R, S32>(p, pinv_sh, valid, K, n, 1, a);
+    for (int r = 0; r < R; ++r) any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+    if (any) rescan<SIGN, R, S32>(L, S, n, 1, a);
 #pragma unroll
-    for (int r = 0; r < R; ++r) K[r] = step<S32>(K[r], p[r], pinv_sh[r], s);
+    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
   }
 }
 
@@ -469,7 +476,7 @@ __global__ __launch_bounds__(256) void state_kernel(const uint64_t *P, uint32_t 
   if (i >= n) return;
   const uint64_t p = P[i];
   const uint64_t pinv = inv64(p);
-  const uint32_t pinv_sh = lo32(pinv) << (32u - s);
+  const uint32_t pinv_sh = step_multiplier(pinv, s);
   uint64_t K = inv2pow(p, pinv, e);
   for (uint32_t w = 0; w < nwin; ++w) K = step<false>(K, p, pinv_sh, s);
   out[i] = K >= p ? K - p : K;

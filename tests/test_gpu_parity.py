@@ -157,10 +157,9 @@ def test_dense_hit_configs_vs_oracle(seed):
     most lanes of a wave hit: compaction, ring and filter under load."""
     rng = random.Random(2000 + seed)
     pmin = rng.randrange(2**32 + 1, 2**36)
-    s = rng.randrange(1, 33)
-    kmax = max(3, min((pmin - 1) >> (s - 1), rng.randrange(3, 2**24)))
-    while kmax << (s - 1) >= pmin:
-        kmax >>= 1
+    # small windows leave room for a large kmax: hits per prime ~ n-range * kmax / 2p
+    s = rng.randrange(1, 15) if seed < 14 else 32 - seed % 2
+    kmax = max(1, min((pmin - 1) >> (s - 1), rng.randrange(2**12, 2**26)))
     kmin = rng.randrange(1, kmax + 1)
     nmin = rng.randrange(64, 3000)
     nmax = nmin + rng.randrange(1, 20000)
@@ -170,7 +169,8 @@ def test_dense_hit_configs_vs_oracle(seed):
     pr = orc.OrcParams(kstep=2, koffset=1, nstep_ref=s, **kw)
     params = ppsd.manual_params(**kw)
     n = _compare(pr, params, seed, batch=rng.choice([0, 257, 4096]))
-    assert n > 0
+    if s < 15:
+        assert n > 0
 
 
 def test_empty_inputs():
@@ -191,7 +191,7 @@ def test_empty_inputs():
 def test_errors():
     with context(**MANY_N) as ctx:
         with pytest.raises(ppsd.PPSDError) as e:
-            ctx.submit_primes(np.array([MANY_N["pmin"] + 1], dtype=np.uint64))  # even
+            ctx.submit_primes(np.array([MANY_N["pmin"] + 2], dtype=np.uint64))  # even
         assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
         with pytest.raises(ppsd.PPSDError) as e:
             ctx.submit_range(MANY_N["pmin"] - 10, MANY_N["pmin"] + 10)
