@@ -69,7 +69,7 @@ def test_interrupt_and_resume(tmp_path):
     """SIGINT mid-run writes a checkpoint at the last finished chunk; the rerun
     resumes there (main.c:332-372) and the factor file and count/sum end up
     exactly as in one uninterrupted run."""
-    args = ["-p1e12", "-P1003e9", "-k3", "-K9999", "-n100", "-N2000000", "-q", "--chunk", "64M"]
+    args = ["-p1e12", "-P1012e9", "-k3", "-K9999", "-n100", "-N2000000", "-q", "-r", "1", "--chunk", "64M"]
     full = tmp_path / "full"
     part = tmp_path / "part"
     full.mkdir()
@@ -78,8 +78,12 @@ def test_interrupt_and_resume(tmp_path):
     assert rc == 0, err_full
     tail_full = [l for l in err_full.splitlines() if l.startswith("count=")]
 
-    p = subprocess.Popen([EXE] + args, cwd=part, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
-    time.sleep(2.5)
+    p = subprocess.Popen([EXE] + args, cwd=part, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                         bufsize=1)
+    t0 = time.time()
+    for line in p.stdout:  # the first status report ("p=..., ... p/sec") comes after a second
+        if line.startswith("p=") or "p/sec" in line or time.time() - t0 > 60:
+            break
     p.send_signal(signal.SIGINT)
     _, err1 = p.communicate(timeout=120)
     assert "Sieve incomplete" in err1, err1
