@@ -36,6 +36,10 @@ PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 MUL32_PER_STEP = 4
 ALU32_PER_STEP = 11
 R_MUL = 2.0
+# HBM bytes one walk launch moves per prime walked, from the FETCH_SIZE and
+# WRITE_SIZE counters (scaled by a 256 MiB copy kernel's readings) on a
+# 2,088,811-prime launch: 16.0 MiB fetched + 4.6 KiB written.
+TRAFFIC_BYTES_PER_PRIME = (16.0 * 2**20 + 4.6 * 2**10) / 2088811
 
 
 def parse():
@@ -140,7 +144,7 @@ def main():
             "peak": PEAK_TOPS,
             "unit": "TOP/s",
             "frac": achieved / PEAK_TOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
         },
     }
     if not a.no_cpu_baseline:
