@@ -139,6 +139,43 @@ int ppsd_close(ppsd_ctx *ctx);
 /* Text of the last error on this thread. */
 const char *ppsd_last_error(void);
 
+/* ---- Batched search (no counterpart in the reference, whose device layer
+ * runs one search per process: pps/appcu.cu:1205-1308).  For callers with
+ * many independent searches -- sieving farms handing out small work units,
+ * the serving loop -- that would each leave most of the GPU idle.
+ *
+ * ppsd_batch_run searches every unit completely: each is a full ppsd_params
+ * (its own p-, k- and n-range, sign, window, k filter) accepted by ppsd_open,
+ * with no partial sieve (qmax 0 or >= isqrt(pmax-1)).  The units' primes are
+ * sieved on the device into one pool; one persistent kernel then drains a
+ * device queue of (unit, 64*r primes, n-segment) tasks, so that the whole
+ * GPU works on whatever mix of units is present.  Results are those of
+ * searching each unit alone with ppsd_submit_range: the same factors (or,
+ * raw != 0, the same divisors before test_factor) -- the order differs.
+ * Synchronous; the context must have no outstanding submission and must not
+ * have been opened with a partial sieve.  Counters go to ppsd_get_stats. */
+int ppsd_batch_run(ppsd_ctx *ctx, const ppsd_params *units, uint32_t nunits, int raw);
+
+/* Hits of the last ppsd_batch_run: up to cap of them to out, with the index
+ * of their unit in unit_of (either may be NULL); *n_out = their number. */
+int ppsd_batch_results(ppsd_ctx *ctx, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out);
+
+/* Per unit of the last ppsd_batch_run: primes walked and their sum mod 2^64
+ * (the run's checksum), for the first nunits units. */
+int ppsd_batch_unit_primes(const ppsd_ctx *ctx, uint64_t *count, uint64_t *sum, uint32_t nunits);
+
+/* Shape of the batched kernel: r primes per lane (1, 2), u windows per
+ * pre-test chunk (4, 8, 16), resident waves per SIMD (1..16), and the
+ * n-segment length in windows, chosen per run between seg_min and seg_max
+ * (seg_min >= 16).  0 keeps a value. */
+int ppsd_batch_set_tuning(ppsd_ctx *ctx, int r, int u, int waves_per_simd, uint32_t seg_min, uint32_t seg_max);
+int ppsd_batch_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *waves_per_simd, uint32_t *seg_min,
+                          uint32_t *seg_max);
+
+/* Raw-hit ring capacity of the batched walk (1 .. 2^22); overflowing task
+ * sets are re-run in halves, so results never depend on it. */
+int ppsd_batch_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
+
 /* Arithmetic check (no counterpart in the reference): out[i] = 2^-e mod P[i]
  * after nwin further windows of s, i.e. 2^-(e + nwin*s) mod P[i], computed by
  * the same device code as the walk.  Requires P[i] odd in (2^32, 2^62),

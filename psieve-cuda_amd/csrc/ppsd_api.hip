@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ppsd.h"
+#include "ppsd_batch.h"
 #include "ppsd_kernels.h"
 
 using namespace ppsd;
@@ -85,6 +86,7 @@ struct ppsd_ctx {
   uint32_t *d_sp = nullptr;     // sieving primes <= qmax
   uint32_t nsp = 0, nsp_small = 0;
   uint64_t qmax = 0;
+  std::vector<uint32_t> sp_sample;  // d_sp[0], d_sp[4096], ... (batched walk)
   uint32_t *d_bitmap = nullptr;
   uint32_t *d_blockcount = nullptr;
   uint64_t nbits_max = 0;
@@ -103,6 +105,30 @@ struct ppsd_ctx {
   bool have_result = false;
 
   ppsd_stats st{};
+
+  // batched walk (allocated on first use)
+  bool b_ready = false;
+  uint64_t b_pool_cap = 0;
+  uint64_t *b_pool = nullptr;
+  uint32_t *b_pool_count = nullptr;
+  unsigned long long *b_pool_sum = nullptr;
+  uint32_t b_piece_cap = 0, b_task_cap = 0;
+  PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
+  Task *b_tasks = nullptr, *h_tasks = nullptr;
+  uint32_t *b_ntasks = nullptr, *b_next = nullptr, *h_ntasks = nullptr;
+  BHit *b_hits = nullptr, *b_kept = nullptr;
+  uint64_t b_hit_cap = 0, b_hit_alloc = 0;
+  unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
+  uint32_t *b_off = nullptr;
+  unsigned long long *b_sums = nullptr;
+  hipGraph_t b_graph = nullptr;
+  hipGraphExec_t b_exec = nullptr;
+  int b_graph_key = -1;
+  int b_r = 1, b_u = 8, b_waves = 8;
+  uint32_t b_seg_min = 2048, b_seg_max = 65536;
+  std::vector<BHit> b_result;
+  std::vector<uint64_t> b_unit_primes, b_unit_sums;
+  uint32_t b_nunits = 0;
 };
 
 extern "C" {
@@ -166,6 +192,17 @@ int ppsd_device_count(void) {
 
 const char *ppsd_last_error(void) { return g_err.c_str(); }
 
+// What the walk itself needs (ppsd_derive_params gives a subset of this):
+// one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
+// nmin >= 64 for the 2^-nmin set-up.
+static bool walk_domain_ok(const ppsd_params *p) {
+  return !(p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) ||
+           p->pmax > ((uint64_t)1 << 62) || p->pmin >= p->pmax || p->nmin < 64 || p->nmin > p->nmax ||
+           p->kmin < 1 || p->kmin > p->kmax || (p->kmax >> (64 - p->nstep)) != 0 ||
+           (p->kmax << (p->nstep - 1)) >= p->pmin || p->sign < -1 || p->sign > 1 || p->kstep < 1 ||
+           p->koffset >= p->kstep);
+}
+
 static int free_ctx(ppsd_ctx *c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
@@ -180,6 +217,23 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_bitmap);
   (void)hipFree(c->d_blockcount);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
+  if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
+  (void)hipFree(c->b_pool);
+  (void)hipFree(c->b_pool_count);
+  (void)hipFree(c->b_pool_sum);
+  (void)hipFree(c->b_pieces);
+  (void)hipFree(c->b_tasks);
+  (void)hipFree(c->b_ntasks);
+  (void)hipFree(c->b_next);
+  (void)hipFree(c->b_hits);
+  (void)hipFree(c->b_kept);
+  (void)hipFree(c->b_counts);
+  (void)hipFree(c->b_off);
+  (void)hipFree(c->b_sums);
+  if (c->h_pieces) (void)hipHostFree(c->h_pieces);
+  if (c->h_tasks) (void)hipHostFree(c->h_tasks);
+  if (c->h_ntasks) (void)hipHostFree(c->h_ntasks);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -234,13 +288,7 @@ static int build_sieving_primes(ppsd_ctx *c) {
 int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) {
   if (!p || !out) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
   *out = nullptr;
-  // What the walk itself needs (ppsd_derive_params gives a subset of this):
-  // one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
-  // nmin >= 64 for the 2^-nmin set-up.
-  if (p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) ||
-      p->pmax > ((uint64_t)1 << 62) || p->pmin >= p->pmax || p->nmin < 64 || p->nmin > p->nmax || p->kmin < 1 ||
-      p->kmin > p->kmax || (p->kmax >> (64 - p->nstep)) != 0 || (p->kmax << (p->nstep - 1)) >= p->pmin ||
-      p->sign < -1 || p->sign > 1 || p->kstep < 1 || p->koffset >= p->kstep)
+  if (!walk_domain_ok(p))
     return set_err(PPSD_ERR_INVALID_PARAM, "parameters outside the walk's domain (use ppsd_derive_params)");
   int ndev = ppsd_device_count();
   if (device < 0 || device >= ndev) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d not available (%d devices)", device, ndev);
@@ -632,6 +680,396 @@ int ppsd_collect_raw(ppsd_ctx *c, ppsd_factor *out, uint64_t cap, uint64_t *n_ou
 }
 
 int ppsd_close(ppsd_ctx *c) { return free_ctx(c); }
+
+// ---------------------------------------------------------------------------
+// batched walk: many work units per launch
+// ---------------------------------------------------------------------------
+
+static int batch_alloc(ppsd_ctx *c) {
+  if (c->b_ready) return 0;
+  c->b_pool_cap = (uint64_t)1 << 26;
+  c->b_piece_cap = 1u << 16;
+  c->b_task_cap = 1u << 22;
+  c->b_hit_cap = c->b_hit_alloc = (uint64_t)1 << 22;
+  HIP_TRY(hipMalloc(&c->b_pool, c->b_pool_cap * sizeof(uint64_t)));
+  HIP_TRY(hipMalloc(&c->b_pool_count, sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_pool_sum, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&c->b_pieces, c->b_piece_cap * sizeof(PieceDesc)));
+  HIP_TRY(hipMalloc(&c->b_tasks, (size_t)c->b_task_cap * sizeof(Task)));
+  HIP_TRY(hipMalloc(&c->b_ntasks, sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_next, sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_hits, c->b_hit_alloc * sizeof(BHit)));
+  HIP_TRY(hipMalloc(&c->b_kept, c->b_hit_alloc * sizeof(BHit)));
+  HIP_TRY(hipMalloc(&c->b_counts, 2 * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&c->b_off, (c->b_piece_cap + 1) * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_sums, (c->b_piece_cap + 1) * sizeof(unsigned long long)));
+  HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
+  c->b_ready = true;
+  return 0;
+}
+
+// Sieving primes up to isqrt(pmax-1) for the largest unit of a batch, and a
+// host sample of every kSpSample-th of them to bound each piece's share.
+constexpr uint32_t kSpSample = 4096;
+
+static int ensure_sieving(ppsd_ctx *c, uint64_t pmax) {
+  const uint64_t need = isqrt64(pmax - 1);
+  if (need > c->qmax) {
+    (void)hipFree(c->d_sp);
+    c->d_sp = nullptr;
+    c->qmax = need;
+    c->sp_sample.clear();
+    int rc = build_sieving_primes(c);
+    if (rc) return rc;
+  }
+  const uint32_t ns = (c->nsp + kSpSample - 1) / kSpSample;
+  if (c->sp_sample.size() != ns) {
+    c->sp_sample.resize(ns);
+    if (ns)
+      HIP_TRY(hipMemcpy2D(c->sp_sample.data(), sizeof(uint32_t), c->d_sp, kSpSample * sizeof(uint32_t),
+                          sizeof(uint32_t), ns, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// Sieving primes a chunk below hi needs (q <= isqrt(hi-1)), rounded up to
+// the sample grid; any prime q < hi's chunk start is harmless to sieve with.
+static uint32_t sieve_count(const ppsd_ctx *c, uint64_t hi) {
+  const uint64_t lim = isqrt64(hi - 1);
+  const auto it = std::upper_bound(c->sp_sample.begin(), c->sp_sample.end(), (uint32_t)std::min<uint64_t>(lim, ~0u));
+  const uint64_t n = (uint64_t)(it - c->sp_sample.begin()) * kSpSample;
+  return (uint32_t)std::min<uint64_t>(n, c->nsp);
+}
+
+static BatchArgs batch_args(ppsd_ctx *c) {
+  BatchArgs a{};
+  a.pool = c->b_pool;
+  a.pieces = c->b_pieces;
+  a.tasks = c->b_tasks;
+  a.ntasks = c->b_ntasks;
+  a.next_task = c->b_next;
+  a.hits = c->b_hits;
+  a.hit_count = c->b_counts;
+  a.hit_cap = c->b_hit_cap;
+  a.kept = c->b_kept;
+  a.kept_count = c->b_counts + 1;
+  a.kept_cap = c->b_hit_cap;
+  a.small_primes = c->d_small;
+  a.nsmall = c->nsmall;
+  return a;
+}
+
+static uint32_t persist_grid(const ppsd_ctx *c) {
+  hipDeviceProp_t prop;
+  int cus = 256;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
+  return (uint32_t)cus * (uint32_t)c->b_waves;  // 256-thread groups: b_waves waves per SIMD
+}
+
+// The round's device sequence -- zero the counters and the queue head, drain
+// the queue, filter -- as one graph, captured once per kernel shape.
+static int round_graph(ppsd_ctx *c) {
+  const int key = c->b_r * 1000000 + c->b_u * 1000 + c->b_waves;
+  if (c->b_exec && c->b_graph_key == key) return 0;
+  if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
+  if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
+  c->b_exec = nullptr;
+  c->b_graph = nullptr;
+  const BatchArgs a = batch_args(c);
+  const uint32_t grid = persist_grid(c);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_zero_u64(c->b_counts, 2, c->stream);
+  if (e == hipSuccess) e = launch_zero_u32(c->b_next, 1, c->stream);
+  if (e == hipSuccess) e = launch_persist(a, c->b_r, c->b_u, grid, c->stream);
+  if (e == hipSuccess) e = launch_batch_filter(a, 1024, c->stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error capturing the round: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error ending capture: %s", hipGetErrorString(e2));
+  c->b_graph = g;
+  HIP_TRY(hipGraphInstantiate(&c->b_exec, g, nullptr, nullptr, 0));
+  c->b_graph_key = key;
+  return 0;
+}
+
+struct Piece {
+  uint32_t unit;
+  uint64_t lo, hi;  // [lo, hi)
+  double est;       // expected primes
+};
+
+// Tasks of one round over the sieved pieces; the segment length halves from
+// b_seg_max down to b_seg_min until the queue holds enough tasks to keep
+// every wave slot busy through the tail; largest tasks first.
+static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> &par, uint32_t npieces,
+                            std::vector<Task> &tasks) {
+  const uint32_t per = 64u * (uint32_t)c->b_r;
+  const uint64_t target = (uint64_t)persist_grid(c) * 4 * 8;  // waves resident x 8
+  uint32_t seg = c->b_seg_max;
+  for (;;) {
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < npieces; ++i) {
+      const PieceDesc &P = c->h_pieces[i];
+      n += (uint64_t)((P.nprimes + per - 1) / per) * ((P.nwin + seg - 1) / seg);
+    }
+    if (n >= target || seg <= c->b_seg_min) break;
+    seg /= 2;
+  }
+  tasks.clear();
+  for (uint32_t i = 0; i < npieces; ++i) {
+    const PieceDesc &P = c->h_pieces[i];
+    for (uint32_t f = 0; f < P.nprimes; f += per)
+      for (uint32_t w = 0; w < P.nwin; w += seg)
+        tasks.push_back(Task{i, f, std::min(per, P.nprimes - f), w, std::min(seg, P.nwin - w)});
+  }
+  std::stable_sort(tasks.begin(), tasks.end(),
+                   [](const Task &x, const Task &y) { return (uint64_t)x.count * x.nw > (uint64_t)y.count * y.nw; });
+  (void)par;
+  return seg;
+}
+
+// Run tasks [0, n) of h_tasks through the round graph; re-run halves when
+// the raw ring overflows.  Appends kept (or raw) hits to b_result.
+static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw) {
+  std::vector<std::pair<uint32_t, uint32_t>> todo{{0, n}};
+  while (!todo.empty()) {
+    auto [f, k] = todo.back();
+    todo.pop_back();
+    if (k == 0) continue;
+    int rc = round_graph(c);
+    if (rc) return rc;
+    memcpy(c->h_tasks, tasks + f, (size_t)k * sizeof(Task));
+    *c->h_ntasks = k;
+    HIP_TRY(hipMemcpyAsync(c->b_tasks, c->h_tasks, (size_t)k * sizeof(Task), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->b_ntasks, c->h_ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    unsigned long long cnt[2];
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(hipGraphLaunch(c->b_exec, c->stream));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipMemcpyAsync(cnt, c->b_counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
+    c->st.walk_launches++;
+    if (cnt[0] > c->b_hit_cap) {
+      c->st.retries++;
+      if (k == 1) {
+        // one task (at most 64*R primes) needs more room than the ring
+        return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "one task has %llu hits (ring %llu)", cnt[0],
+                       (unsigned long long)c->b_hit_cap);
+      }
+      todo.push_back({f + k / 2, k - k / 2});
+      todo.push_back({f, k / 2});
+      continue;
+    }
+    const uint64_t take = raw ? cnt[0] : cnt[1];
+    const size_t old = c->b_result.size();
+    c->b_result.resize(old + take);
+    if (take)
+      HIP_TRY(hipMemcpy(c->b_result.data() + old, raw ? c->b_hits : c->b_kept, take * sizeof(BHit),
+                        hipMemcpyDeviceToHost));
+    c->st.raw_hits += cnt[0];
+    c->st.factors += cnt[1];
+  }
+  return 0;
+}
+
+// One round: sieve the pieces into the pool, build and drain the queue.
+// Returns 1 (no error) when the pool overflowed and the caller must split.
+static int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+                     const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit) {
+  const uint32_t np = (uint32_t)(e - b);
+  HIP_TRY(launch_zero_u32(c->b_pool_count, 1, c->stream));
+  HIP_TRY(launch_zero_u64(c->b_pool_sum, 1, c->stream));
+  HIP_TRY(launch_snapshot(c->b_pool_count, c->b_pool_sum, c->b_off, c->b_sums, 0, c->stream));
+  HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  for (uint32_t i = 0; i < np; ++i) {
+    const Piece &P = pieces[b + i];
+    for (uint64_t lo = P.lo | 1; lo < P.hi;) {
+      uint64_t nbits = (P.hi - lo + 1) / 2;
+      nbits = std::min<uint64_t>(nbits, c->nbits_max);
+      SieveArgs sa{};
+      sa.lo = lo;
+      sa.nbits = nbits;
+      sa.q = c->d_sp;
+      sa.nq = sieve_count(c, lo + 2 * nbits);
+      sa.nq_small = std::min(c->nsp_small, sa.nq);
+      sa.bitmap = c->d_bitmap;
+      HIP_TRY(launch_sieve(sa, c->stream));
+      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->b_pool, (uint32_t)c->b_pool_cap,
+                               c->b_pool_count, c->b_pool_sum, c->stream));
+      lo += 2 * nbits;
+    }
+    HIP_TRY(launch_snapshot(c->b_pool_count, c->b_pool_sum, c->b_off, c->b_sums, i + 1, c->stream));
+  }
+  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  std::vector<uint32_t> off(np + 1);
+  std::vector<unsigned long long> sums(np + 1);
+  HIP_TRY(hipMemcpyAsync(off.data(), c->b_off, (np + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(sums.data(), c->b_sums, (np + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);
+  if (off[np] > c->b_pool_cap) return 1;
+  for (uint32_t i = 0; i < np; ++i) {
+    const Piece &P = pieces[b + i];
+    const ppsd_params &u = *par[P.unit];
+    PieceDesc &D = c->h_pieces[i];
+    D.kmin = u.kmin;
+    D.kmax = u.kmax;
+    D.nmin = u.nmin;
+    D.nmax = u.nmax;
+    D.s = u.nstep;
+    D.nwin = windows(u);
+    D.sign = u.sign;
+    D.kstep = u.kstep;
+    D.koffset = u.koffset;
+    D.prime_off = off[i];
+    D.nprimes = off[i + 1] - off[i];
+    D.unit = P.unit;
+    piece_unit[i] = P.unit;
+    c->b_unit_primes[P.unit] += D.nprimes;
+    c->b_unit_sums[P.unit] += sums[i + 1] - sums[i];
+    c->st.primes += D.nprimes;
+    c->st.prime_sum += sums[i + 1] - sums[i];
+    c->st.window_steps += (uint64_t)D.nprimes * D.nwin * (D.sign == 0 ? 2 : 1);
+  }
+  HIP_TRY(hipMemcpyAsync(c->b_pieces, c->h_pieces, np * sizeof(PieceDesc), hipMemcpyHostToDevice, c->stream));
+  std::vector<Task> tasks;
+  build_tasks(c, par, np, tasks);
+  const size_t first_hit = c->b_result.size();
+  for (size_t t = 0; t < tasks.size(); t += c->b_task_cap) {
+    const uint32_t k = (uint32_t)std::min<size_t>(c->b_task_cap, tasks.size() - t);
+    int rc = run_tasks(c, tasks.data() + t, k, raw);
+    if (rc) return rc;
+  }
+  // piece index -> caller's unit
+  for (size_t h = first_hit; h < c->b_result.size(); ++h) c->b_result[h].piece = piece_unit[c->b_result[h].piece];
+  return 0;
+}
+
+int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int raw) {
+  if (!c || (!units && nunits)) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "a submission is outstanding");
+  std::vector<const ppsd_params *> par(nunits);
+  uint64_t pmax_all = 0;
+  for (uint32_t u = 0; u < nunits; ++u) {
+    if (!walk_domain_ok(&units[u]))
+      return set_err(PPSD_ERR_INVALID_PARAM, "unit %u: parameters outside the walk's domain", u);
+    if (units[u].qmax && units[u].qmax < isqrt64(units[u].pmax - 1))
+      return set_err(PPSD_ERR_INVALID_PARAM, "unit %u: a batch sieves completely (qmax must be 0)", u);
+    par[u] = &units[u];
+    pmax_all = std::max(pmax_all, units[u].pmax);
+  }
+  if (c->prm.qmax && c->qmax < isqrt64(c->prm.pmax - 1))
+    return set_err(PPSD_ERR_STATE, "context opened with a partial sieve (qmax)");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = batch_alloc(c);
+  if (rc) return rc;
+  if (nunits) {
+    rc = ensure_sieving(c, pmax_all);
+    if (rc) return rc;
+  }
+  c->b_result.clear();
+  c->b_unit_primes.assign(nunits, 0);
+  c->b_unit_sums.assign(nunits, 0);
+  c->b_nunits = nunits;
+  // pieces of at most a quarter of the pool (expected primes)
+  std::vector<Piece> pieces;
+  const double piece_max = (double)c->b_pool_cap / 4;
+  for (uint32_t u = 0; u < nunits; ++u) {
+    const uint64_t lo = units[u].pmin, hi = units[u].pmax;
+    const double dens = 1.05 / log((double)lo);
+    const uint64_t width = std::max<uint64_t>(2, (uint64_t)(piece_max / dens) & ~(uint64_t)1);
+    for (uint64_t a = lo; a < hi; a += width) {
+      const uint64_t z = std::min(hi, a + width);
+      pieces.push_back(Piece{u, a, z, (double)(z - a) * dens + 64});
+    }
+  }
+  // rounds: pieces packed in order up to 90% of the pool
+  std::vector<uint32_t> piece_unit(c->b_piece_cap);
+  size_t b = 0;
+  while (b < pieces.size()) {
+    size_t e = b;
+    double est = 0;
+    while (e < pieces.size() && e - b < c->b_piece_cap && (e == b || est + pieces[e].est <= 0.9 * c->b_pool_cap))
+      est += pieces[e++].est;
+    rc = run_round(c, pieces, b, e, par, raw != 0, piece_unit);
+    if (rc == 1) {  // pool overflow: fewer pieces this round
+      if (e - b == 1) return set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
+      rc = run_round(c, pieces, b, b + (e - b) / 2, par, raw != 0, piece_unit);
+      if (rc) return rc < 0 ? rc : set_err(PPSD_ERR_FAILURE, "prime pool overflow");
+      b += (e - b) / 2;
+      continue;
+    }
+    if (rc) return rc;
+    b = e;
+  }
+  return 0;
+}
+
+int ppsd_batch_results(ppsd_ctx *c, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (n_out) *n_out = c->b_result.size();
+  const uint64_t n = std::min<uint64_t>(cap, c->b_result.size());
+  for (uint64_t i = 0; i < n; ++i) {
+    const BHit &h = c->b_result[i];
+    if (out) out[i] = ppsd_factor{h.p, h.k, h.n, h.c};
+    if (unit_of) unit_of[i] = h.piece;
+  }
+  return 0;
+}
+
+int ppsd_batch_unit_primes(const ppsd_ctx *c, uint64_t *count, uint64_t *sum, uint32_t nunits) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (nunits > c->b_nunits) return set_err(PPSD_ERR_INVALID_PARAM, "only %u units in the last batch", c->b_nunits);
+  for (uint32_t u = 0; u < nunits; ++u) {
+    if (count) count[u] = c->b_unit_primes[u];
+    if (sum) sum[u] = c->b_unit_sums[u];
+  }
+  return 0;
+}
+
+int ppsd_batch_set_tuning(ppsd_ctx *c, int r, int u, int waves_per_simd, uint32_t seg_min, uint32_t seg_max) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (r && r != 1 && r != 2) return set_err(PPSD_ERR_INVALID_PARAM, "r must be 1 or 2");
+  if (u && u != 4 && u != 8 && u != 16) return set_err(PPSD_ERR_INVALID_PARAM, "u must be 4, 8 or 16");
+  if (waves_per_simd && (waves_per_simd < 1 || waves_per_simd > 16))
+    return set_err(PPSD_ERR_INVALID_PARAM, "waves per SIMD must be 1..16");
+  if ((seg_min && seg_min < 16) || (seg_max && seg_min && seg_max < seg_min))
+    return set_err(PPSD_ERR_INVALID_PARAM, "segment lengths: 16 <= seg_min <= seg_max");
+  if (r) c->b_r = r;
+  if (u) c->b_u = u;
+  if (waves_per_simd) c->b_waves = waves_per_simd;
+  if (seg_min) c->b_seg_min = seg_min;
+  if (seg_max) c->b_seg_max = seg_max;
+  if (c->b_seg_max < c->b_seg_min) c->b_seg_max = c->b_seg_min;
+  return 0;
+}
+
+int ppsd_batch_get_tuning(const ppsd_ctx *c, int *r, int *u, int *waves_per_simd, uint32_t *seg_min,
+                          uint32_t *seg_max) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (r) *r = c->b_r;
+  if (u) *u = c->b_u;
+  if (waves_per_simd) *waves_per_simd = c->b_waves;
+  if (seg_min) *seg_min = c->b_seg_min;
+  if (seg_max) *seg_max = c->b_seg_max;
+  return 0;
+}
+
+int ppsd_batch_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  int rc = batch_alloc(c);
+  if (rc) return rc;
+  if (cap < 1 || cap > c->b_hit_alloc)
+    return set_err(PPSD_ERR_INVALID_PARAM, "capacity must be 1..%llu", (unsigned long long)c->b_hit_alloc);
+  c->b_hit_cap = cap;
+  c->b_graph_key = -1;  // the graph holds the capacity in its kernel arguments
+  return 0;
+}
 
 int ppsd_debug_state(int device, const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s,
                      uint64_t *out) {

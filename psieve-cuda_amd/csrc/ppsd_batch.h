@@ -1,0 +1,56 @@
+// ppsd_batch.h -- device data of the batched (many work units) walk.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppsd {
+
+// One piece (p-subrange) of a work unit, as the persistent kernel sees it.
+struct PieceDesc {
+  uint64_t kmin, kmax;
+  uint32_t nmin, nmax;  // nmax exclusive
+  uint32_t s, nwin;     // window and windows over [nmin, nmax)
+  int32_t sign;         // +1, -1, 0 both
+  uint32_t kstep, koffset;
+  uint32_t prime_off, nprimes;  // its primes in the pool
+  uint32_t unit;                // caller's unit index
+};
+
+// A unit of the work queue: count (<= 64*R) primes from pool offset
+// pieces[piece].prime_off + first, windows w0 .. w0+nw of that piece.
+struct Task {
+  uint32_t piece, first, count, w0, nw;
+};
+
+// Raw hit of the batched walk.
+struct BHit {
+  uint64_t p;
+  uint64_t k;
+  uint32_t n;
+  int32_t c;
+  uint32_t piece;
+  uint32_t pad;
+};
+
+struct BatchArgs {
+  const uint64_t *pool;
+  const PieceDesc *pieces;
+  const Task *tasks;
+  const uint32_t *ntasks;  // device: tasks in this round
+  uint32_t *next_task;     // device: queue head, zeroed before each round
+  BHit *hits;
+  unsigned long long *hit_count;
+  uint64_t hit_cap;
+  BHit *kept;
+  unsigned long long *kept_count;
+  uint64_t kept_cap;
+  const uint32_t *small_primes;
+  uint32_t nsmall;
+};
+
+hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipStream_t st);
+hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st);
+hipError_t launch_snapshot(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
+                           unsigned long long *sums, uint32_t i, hipStream_t st);
+
+}  // namespace ppsd

@@ -1,0 +1,143 @@
+// ppsd_batch.hip -- many independent work units in one persistent launch.
+//
+// A work unit is a full search (its own p-range, k-range, n-range, sign and
+// window).  Units are cut into pieces (p-subranges) that fit the device prime
+// pool; a round sieves its pieces into the pool, then a device work queue of
+// tasks -- (piece, block of 64*R primes, segment of windows) -- is drained by
+// a persistent kernel whose waves pull tasks with one atomic each until the
+// queue is empty.  Splitting n-ranges into segments is what keeps the whole
+// chip busy on units with few primes; the segment start state is recomputed
+// with one 2^-e powmod per lane.  The round's zeroing, walk and filter are
+// captured once as a hipGraph and replayed for every round.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ppsd.h"
+#include "ppsd_batch.h"
+#include "ppsd_filter.h"
+#include "ppsd_walk.h"
+
+namespace ppsd {
+
+// Raw-hit ring of the batched walk: 32-byte records tagged with the piece.
+struct PieceSink {
+  BHit *hits;
+  unsigned long long *count;
+  uint64_t cap;
+  uint32_t piece;
+  __device__ __forceinline__ void put(uint64_t idx, uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
+    BHit h;
+    h.p = p;
+    h.k = k;
+    h.n = n;
+    h.c = c;
+    h.piece = piece;
+    h.pad = 0;
+    hits[idx] = h;
+  }
+};
+
+template <int SIGN, int R, int U, bool S32>
+__device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P, const Task &T) {
+  LanePrimes<R> L;
+  LaneStates<R> S;
+  const uint32_t first = T.first + __lane_id() * R;
+  const uint32_t end = T.first + T.count;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    L.valid[r] = first + r < end;
+    L.p[r] = a.pool[P.prime_off + (L.valid[r] ? first + r : T.first)];
+  }
+  const uint32_t n0 = P.nmin + T.w0 * P.s;
+  start_states<SIGN, R>(L, S, n0, P.s);
+  const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s};
+  const PieceSink sink{a.hits, a.hit_count, a.hit_cap, T.piece};
+  walk_windows<SIGN, R, U, S32>(L, S, n0, T.nw, rule, sink);
+}
+
+// Every wave pulls tasks until the queue is empty; all waves reach the exit.
+template <int R, int U>
+__global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
+  const uint32_t ntasks = *a.ntasks;
+  for (;;) {
+    uint32_t t = 0;
+    if (__lane_id() == 0) t = atomicAdd(a.next_task, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    if (t >= ntasks) return;
+    const Task T = a.tasks[t];
+    const PieceDesc P = a.pieces[T.piece];
+    const bool s32 = P.s == 32;
+    if (P.sign == 1) {
+      if (s32) run_task<1, R, U, true>(a, P, T); else run_task<1, R, U, false>(a, P, T);
+    } else if (P.sign == -1) {
+      if (s32) run_task<-1, R, U, true>(a, P, T); else run_task<-1, R, U, false>(a, P, T);
+    } else {
+      if (s32) run_task<0, R, U, true>(a, P, T); else run_task<0, R, U, false>(a, P, T);
+    }
+  }
+}
+
+// test_factor on every raw hit of the round, one wave per hit, with the
+// rule of the hit's piece.
+__global__ __launch_bounds__(256) void batch_filter_kernel(BatchArgs a) {
+  const unsigned long long raw = *a.hit_count;
+  const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = wave; h < nraw; h += nwaves) {
+    const BHit x = a.hits[h];
+    const PieceDesc P = a.pieces[x.piece];
+    const bool keep = keep_factor(x.k, x.n, x.c, P.kstep, P.koffset, P.nmin, P.nmax, a.small_primes, a.nsmall);
+    if (keep && __lane_id() == 0) {
+      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
+      if (idx < a.kept_cap) a.kept[idx] = x;
+    }
+  }
+}
+
+// Copies the running prime count and sum after a piece was sieved.
+__global__ void snapshot_kernel(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
+                                unsigned long long *sums, uint32_t i) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    off[i] = *count;
+    sums[i] = *sum;
+  }
+}
+
+template <int R, int U>
+static hipError_t launch_persist_u(const BatchArgs &a, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL((persist_walk_kernel<R, U>), dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipStream_t st) {
+  if (r == 1) {
+    if (u == 4) return launch_persist_u<1, 4>(a, grid, st);
+    if (u == 8) return launch_persist_u<1, 8>(a, grid, st);
+    if (u == 16) return launch_persist_u<1, 16>(a, grid, st);
+  } else if (r == 2) {
+    if (u == 4) return launch_persist_u<2, 4>(a, grid, st);
+    if (u == 8) return launch_persist_u<2, 8>(a, grid, st);
+    if (u == 16) return launch_persist_u<2, 16>(a, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(batch_filter_kernel, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_snapshot(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
+                           unsigned long long *sums, uint32_t i, hipStream_t st) {
+  hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, st, count, sum, off, sums, i);
+  return hipGetLastError();
+}
+
+}  // namespace ppsd

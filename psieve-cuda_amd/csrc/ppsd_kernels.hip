@@ -9,99 +9,15 @@
 //                    (replaces sieve.c + main.c:750-777)
 //   state_kernel     2^-e mod p then nwin windows; arithmetic check only
 #include "ppsd_arith.h"
+#include "ppsd_filter.h"
 #include "ppsd_kernels.h"
+#include "ppsd_walk.h"
 
 namespace ppsd {
 
 // ---------------------------------------------------------------------------
-// hit emission: ballot + prefix inside the wave, one atomic per wave
+// plain walk: one lane per R primes of one batch
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void emit(bool hit, uint64_t p, uint64_t k, uint32_t n, int32_t c,
-                                     const WalkArgs &a) {
-  const uint64_t mask = __ballot(hit);
-  if (mask == 0) return;
-  const uint32_t lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  unsigned long long base = 0;
-  if ((int)lane == leader) base = atomicAdd(a.hit_count, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
-  if (hit) {
-    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-    const uint64_t idx = base + (uint64_t)__popcll(below);
-    if (idx < a.hit_cap) {
-      Hit h;
-      h.p = p;
-      h.k = k;
-      h.n = n;
-      h.c = c;
-      a.hits[idx] = h;
-    }
-  }
-}
-
-// Exact window test (pps/app.c:1093-1109 semantics) on a state reduced to
-// [0, p): K = k*2^i with i < s, kmin <= k <= kmax, n+i inside [nmin, nmax).
-__device__ __forceinline__ bool exact(uint64_t K, uint32_t n, const WalkArgs &a, uint64_t &k, uint32_t &nn) {
-  const uint32_t lo = lo32(K);
-  if (lo == 0) return false;
-  const uint32_t i = (uint32_t)__builtin_ctz(lo);
-  if (i >= a.s) return false;
-  k = K >> i;
-  nn = n + i;
-  return k <= a.kmax && k >= a.kmin && nn < a.nmax;
-}
-
-// Pre-test of one window: a state not yet reduced (K >= p), or an odd part
-// <= kmax for the walked sign (both signs: also for p - K).
-template <int SIGN>
-__device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
-  const bool unreduced = K >= p;
-  if (SIGN == 0) return (int)unreduced | (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
-  return (int)unreduced | (int)pretest(K, kmax);
-}
-
-// Per-lane constants of the R primes a lane walks, and a snapshot of their
-// states; passed by value so the rare re-walk call needs no scratch memory.
-template <int R>
-struct LanePrimes {
-  uint64_t p[R];
-  uint32_t ps[R];
-  bool valid[R];
-};
-template <int R>
-struct LaneStates {
-  uint64_t K[R];
-};
-
-// Re-walk `cnt` windows from state K (window start n) with exact tests and
-// emission.  Reached only when a pre-test fired somewhere in the chunk.
-template <int SIGN, int R, bool S32>
-__device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
-                                    const WalkArgs &a) {
-  for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint64_t Kc = S.K[r] >= L.p[r] ? S.K[r] - L.p[r] : S.K[r];
-      uint64_t k = 0;
-      uint32_t nn = 0;
-      if (SIGN == 0) {
-        bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
-        emit(h, L.p[r], k, nn, -1, a);
-        h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
-        emit(h, L.p[r], k, nn, +1, a);
-      } else {
-        const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
-        emit(h, L.p[r], k, nn, SIGN, a);
-      }
-      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
-    }
-  }
-}
-
-// The walk.  Lane g owns primes g*R .. g*R+R-1.  K starts at -c*2^-nmin mod p
-// (for both signs: 2^-nmin, whose negation p-K is the +1 form) and advances
-// one window of s exponents per step.  Steps run in chunks of U; a chunk only
-// leaves the straight-line path when one of its pre-tests fired.
 template <int SIGN, int R, int U, bool S32>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   const uint32_t cnt = a.count_d ? *a.count_d : a.count;
@@ -114,36 +30,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   for (int r = 0; r < R; ++r) {
     L.valid[r] = first + r < count;
     L.p[r] = a.P[L.valid[r] ? first + r : count - 1];
-    const uint64_t pinv = inv64(L.p[r]);
-    L.ps[r] = step_multiplier(pinv, a.s);
-    const uint64_t x = inv2pow(L.p[r], pinv, a.nmin);
-    S.K[r] = (SIGN == 1) ? L.p[r] - x : x;
   }
-  const uint32_t s = a.s;
-  uint32_t n = a.nmin;
-  uint32_t w = 0;
-  for (; w + U <= a.nwin; w += U) {
-    const LaneStates<R> snap = S;
-    bool any = false;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
-        S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
-      }
-    }
-    if (any) rescan<SIGN, R, S32>(L, snap, n, U, a);
-    n += U * s;
-  }
-  for (; w < a.nwin; ++w, n += s) {
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < R; ++r) any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
-    if (any) rescan<SIGN, R, S32>(L, S, n, 1, a);
-#pragma unroll
-    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
-  }
+  start_states<SIGN, R>(L, S, a.nmin, a.s);
+  const WindowRule rule{a.kmin, a.kmax, a.nmax, a.s};
+  const RingSink sink{a.hits, a.hit_count, a.hit_cap};
+  walk_windows<SIGN, R, U, S32>(L, S, a.nmin, a.nwin, rule, sink);
 }
 
 template <int SIGN, int R, bool S32>
@@ -186,81 +77,15 @@ hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid,
 // ---------------------------------------------------------------------------
 // filter: test_factor (pps/app.c:159-186) on every raw hit, one wave per hit
 // ---------------------------------------------------------------------------
-
-// x*y mod q for x, y < q < 2^15: the float quotient estimate is off by at most
-// one (relative error ~2^-22 on a quotient below 2^15), so two corrections
-// make it exact.
-__device__ __forceinline__ uint32_t mulmod_q(uint32_t x, uint32_t y, uint32_t q, float rq) {
-  const uint32_t prod = x * y;
-  const uint32_t e = (uint32_t)((float)prod * rq);
-  int32_t r = (int32_t)(prod - e * q);
-  r += (r < 0) ? (int32_t)q : 0;
-  r -= (r >= (int32_t)q) ? (int32_t)q : 0;
-  return (uint32_t)r;
-}
-
-// q | k*2^n + c ?
-__device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, uint32_t q) {
-  const float rq = __frcp_rn((float)q);
-  uint32_t t = 1;
-  for (int b = 31 - __clz((int)n); b >= 0; --b) {
-    t = mulmod_q(t, t, q, rq);
-    if ((n >> b) & 1u) {
-      t <<= 1;
-      t -= (t >= q) ? q : 0;
-    }
-  }
-  const uint32_t km = (uint32_t)(k % q);
-  const uint32_t v = mulmod_q(km, t, q, rq);
-  return (c > 0) ? (v + 1 == q) : (v == 1);
-}
-
-// The residue pre-tests of test_factor in wrapping 64-bit arithmetic with their
-// k-size guards; mod31 receives (k << (n % 10)) + c when that branch is
-// reached and stays 1 otherwise, as in the reference.
-__device__ __forceinline__ bool pre_rules(uint64_t k, uint32_t n, int32_t c, uint64_t &mod31) {
-  const uint64_t cc = (uint64_t)(int64_t)c;
-  const uint32_t khigh = (uint32_t)(k >> 32);
-  const uint32_t r15 = (uint32_t)(((k << (n & 3)) + cc) % 15);
-  // 1 where r15 is divisible by neither 3 nor 5 (pps/app.c:152)
-  const bool ok15 = (0x6996u >> r15) & 1u;
-  mod31 = 1;
-  bool keep = ok15 && (uint32_t)(((k << (n % 3)) + cc) % 7) != 0;
-  if (keep && khigh < (1u << 24)) {
-    keep = (uint32_t)(((k << (n & 7)) + cc) % 17) != 0;
-    if (keep && khigh < (1u << 22)) {
-      mod31 = (k << (n % 10)) + cc;
-      keep = (uint32_t)(mod31 % 11) != 0;
-      if (keep && khigh < (1u << 21)) {
-        keep = (uint32_t)(((k << (n % 11)) + cc) % 23) != 0;
-        if (keep && khigh < (1u << 20)) {
-          keep = (uint32_t)(((k << (n % 12)) + cc) % 13) != 0;
-          if (keep && khigh < (1u << 14)) keep = (uint32_t)(((k << (n % 18)) + cc) % 19) != 0;
-        }
-      }
-    }
-  }
-  return keep;
-}
-
 __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
-  const uint32_t lane = __lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t h = wave; h < nraw; h += nwaves) {
     const Hit x = a.hits[h];
-    const uint64_t b = x.k / a.kstep;
-    bool keep = x.k == (uint64_t)a.kstep * b + a.koffset && x.n >= a.nmin && x.n < a.nmax;
-    uint64_t mod31 = 1;
-    if (keep) keep = pre_rules(x.k, x.n, x.c, mod31) && (uint32_t)(mod31 % 31) != 0;
-    if (keep) {
-      bool d = false;
-      for (uint32_t j = lane; j < a.nsmall; j += 64) d |= divides(x.k, x.n, x.c, a.small_primes[j]);
-      keep = __ballot(d) == 0;
-    }
-    if (keep && lane == 0) {
+    const bool keep = keep_factor(x.k, x.n, x.c, a.kstep, a.koffset, a.nmin, a.nmax, a.small_primes, a.nsmall);
+    if (keep && __lane_id() == 0) {
       const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
       if (idx < a.kept_cap) a.kept[idx] = x;
     }

@@ -1,0 +1,153 @@
+// ppsd_walk.h -- the window loop shared by the plain and the persistent walk
+// kernels: pre-test, exact test, in-wave hit compaction, chunked stepping.
+#pragma once
+#include "ppsd_arith.h"
+#include "ppsd_kernels.h"
+
+namespace ppsd {
+
+// What a window test needs besides the state.
+struct WindowRule {
+  uint64_t kmin, kmax;
+  uint32_t nmax;  // exclusive
+  uint32_t s;     // window, 1..32
+};
+
+// Raw-hit ring of the plain walk: 24-byte records.
+struct RingSink {
+  Hit *hits;
+  unsigned long long *count;
+  uint64_t cap;
+  __device__ __forceinline__ void put(uint64_t idx, uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
+    Hit h;
+    h.p = p;
+    h.k = k;
+    h.n = n;
+    h.c = c;
+    hits[idx] = h;
+  }
+};
+
+// Ballot + prefix inside the wave, one atomic per wave; slots past the
+// ring's capacity are counted but not written (the host re-walks then).
+template <class Sink>
+__device__ __forceinline__ void emit(bool hit, uint64_t p, uint64_t k, uint32_t n, int32_t c, const Sink &sink) {
+  const uint64_t mask = __ballot(hit);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if ((int)lane == leader) base = atomicAdd(sink.count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (hit) {
+    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    const uint64_t idx = base + (uint64_t)__popcll(below);
+    if (idx < sink.cap) sink.put(idx, p, k, n, c);
+  }
+}
+
+// Exact window test (pps/app.c:1093-1109 semantics) on a state reduced to
+// [0, p): K = k*2^i with i < s, kmin <= k <= kmax, n+i inside [nmin, nmax).
+__device__ __forceinline__ bool exact(uint64_t K, uint32_t n, const WindowRule &a, uint64_t &k, uint32_t &nn) {
+  const uint32_t lo = lo32(K);
+  if (lo == 0) return false;
+  const uint32_t i = (uint32_t)__builtin_ctz(lo);
+  if (i >= a.s) return false;
+  k = K >> i;
+  nn = n + i;
+  return k <= a.kmax && k >= a.kmin && nn < a.nmax;
+}
+
+// Pre-test of one window: a state not yet reduced (K >= p), or an odd part
+// <= kmax for the walked sign (both signs: also for p - K).
+template <int SIGN>
+__device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
+  const bool unreduced = K >= p;
+  if (SIGN == 0) return (int)unreduced | (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
+  return (int)unreduced | (int)pretest(K, kmax);
+}
+
+// Per-lane constants of the R primes a lane walks, and a snapshot of their
+// states; passed by value so the rare re-walk call needs no scratch memory.
+template <int R>
+struct LanePrimes {
+  uint64_t p[R];
+  uint32_t ps[R];
+  bool valid[R];
+};
+template <int R>
+struct LaneStates {
+  uint64_t K[R];
+};
+
+// Re-walk `cnt` windows from state K (window start n) with exact tests and
+// emission.  Reached only when a pre-test fired somewhere in the chunk.
+template <int SIGN, int R, bool S32, class Sink>
+__device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
+                                       const WindowRule &a, const Sink &sink) {
+  for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t Kc = S.K[r] >= L.p[r] ? S.K[r] - L.p[r] : S.K[r];
+      uint64_t k = 0;
+      uint32_t nn = 0;
+      if (SIGN == 0) {
+        bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
+        emit(h, L.p[r], k, nn, -1, sink);
+        h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
+        emit(h, L.p[r], k, nn, +1, sink);
+      } else {
+        const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
+        emit(h, L.p[r], k, nn, SIGN, sink);
+      }
+      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
+    }
+  }
+}
+
+// Walk nwin windows from window start n, states S.  Steps run in chunks of U
+// and a chunk only leaves the straight-line path when one of its pre-tests
+// fired (then it is walked again with exact tests).
+template <int SIGN, int R, int U, bool S32, class Sink>
+__device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
+                                             const WindowRule &a, const Sink &sink) {
+  const uint32_t s = a.s;
+  uint32_t w = 0;
+  for (; w + U <= nwin; w += U) {
+    const LaneStates<R> snap = S;
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+        S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
+      }
+    }
+    if (any) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
+    n += U * s;
+  }
+  for (; w < nwin; ++w, n += s) {
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+    if (any) rescan<SIGN, R, S32>(L, S, n, 1, a, sink);
+#pragma unroll
+    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
+  }
+}
+
+// Start states: K = -c * 2^-e mod p (both signs: 2^-e, whose negation p - K
+// is the +1 form).
+template <int SIGN, int R>
+__device__ __forceinline__ void start_states(LanePrimes<R> &L, LaneStates<R> &S, uint32_t e, uint32_t s) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t pinv = inv64(L.p[r]);
+    L.ps[r] = step_multiplier(pinv, s);
+    const uint64_t x = inv2pow(L.p[r], pinv, e);
+    S.K[r] = (SIGN == 1) ? L.p[r] - x : x;
+  }
+}
+
+}  // namespace ppsd

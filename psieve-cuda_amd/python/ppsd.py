@@ -78,6 +78,13 @@ def _load():
         "ppsd_close": (C.c_int, [C.c_void_p]),
         "ppsd_last_error": (C.c_char_p, []),
         "ppsd_debug_state": (C.c_int, [C.c_int, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
+        "ppsd_batch_run": (C.c_int, [C.c_void_p, _P(Params), C.c_uint32, C.c_int]),
+        "ppsd_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, _P(C.c_uint32), C.c_uint64, _u64p]),
+        "ppsd_batch_unit_primes": (C.c_int, [C.c_void_p, _u64p, _u64p, C.c_uint32]),
+        "ppsd_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
+        "ppsd_batch_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int),
+                                            _P(C.c_uint32), _P(C.c_uint32)]),
+        "ppsd_batch_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -244,6 +251,40 @@ class Context:
             f, _, _ = self.collect_raw() if raw else self.collect()
             parts.append(f)
         return np.concatenate(parts) if parts else np.zeros(0, dtype=FACTOR_DTYPE)
+
+    # ---- batched search: many independent units in one persistent launch
+    def batch_run(self, units, raw: bool = False):
+        """Search every unit (a sequence of Params) completely; returns
+        (hits, unit index per hit, primes per unit, prime sum per unit)."""
+        arr = (Params * max(1, len(units)))(*units)
+        _check(lib().ppsd_batch_run(self._h, arr, len(units), 1 if raw else 0), "ppsd_batch_run")
+        n = C.c_uint64()
+        _check(lib().ppsd_batch_results(self._h, None, None, 0, C.byref(n)), "ppsd_batch_results")
+        out = np.zeros(n.value, dtype=FACTOR_DTYPE)
+        unit = np.zeros(n.value, dtype=np.uint32)
+        if n.value:
+            _check(lib().ppsd_batch_results(self._h, out.ctypes.data_as(C.c_void_p),
+                                            unit.ctypes.data_as(_P(C.c_uint32)), n.value, C.byref(n)),
+                   "ppsd_batch_results")
+        cnt = np.zeros(len(units), dtype=np.uint64)
+        sums = np.zeros(len(units), dtype=np.uint64)
+        if len(units):
+            _check(lib().ppsd_batch_unit_primes(self._h, _ptr(cnt), _ptr(sums), len(units)),
+                   "ppsd_batch_unit_primes")
+        return out, unit, cnt, sums
+
+    def set_batch_tuning(self, r=0, u=0, waves=0, seg_min=0, seg_max=0):
+        _check(lib().ppsd_batch_set_tuning(self._h, r, u, waves, seg_min, seg_max), "ppsd_batch_set_tuning")
+
+    def batch_tuning(self):
+        r, u, w = C.c_int(), C.c_int(), C.c_int()
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(lib().ppsd_batch_get_tuning(self._h, C.byref(r), C.byref(u), C.byref(w), C.byref(a), C.byref(b)),
+               "ppsd_batch_get_tuning")
+        return r.value, u.value, w.value, a.value, b.value
+
+    def set_batch_hit_capacity(self, cap: int):
+        _check(lib().ppsd_batch_set_hit_capacity(self._h, cap), "ppsd_batch_set_hit_capacity")
 
 
 def debug_state(P, e: int, nwin: int, s: int, device: int = 0) -> np.ndarray:

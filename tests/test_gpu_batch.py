@@ -1,0 +1,177 @@
+"""Batched search (ppsd_batch_run): many independent units in one persistent
+launch.  Every unit's result must equal what the oracle (and the plain
+per-unit path, ppsd_submit_range) gives for that unit alone."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import pps_oracle as orc
+import ppsd
+from conftest import GOLDEN, read_lines
+
+pytestmark = pytest.mark.gpu
+
+RUNS = json.load(open(os.path.join(GOLDEN, "runs.json")))
+U64 = 2**64 - 1
+
+
+def golden_set(name):
+    return {orc.parse_line(l) for l in read_lines(name)}
+
+
+def _dense_unit(rng):
+    """A unit with many hits per prime (small windows, large kmax)."""
+    pmin = rng.randrange(2**32 + 1, 2**36)
+    s = rng.randrange(1, 15) if rng.random() < 0.85 else rng.choice((31, 32))
+    kmax = max(1, min((pmin - 1) >> (s - 1), rng.randrange(2**12, 2**26)))
+    kw = dict(pmin=pmin, pmax=pmin + rng.randrange(500, 20000), kmin=rng.randrange(1, kmax + 1), kmax=kmax,
+              nmin=rng.randrange(64, 3000), sign=rng.choice((1, -1, 0)), nstep=s)
+    kw["nmax"] = kw["nmin"] + rng.randrange(0, 20000)
+    ks = rng.choice([(2, 1), (2, 1), (6, 3), (4, 1)])
+    pr = orc.OrcParams(kstep=ks[0], koffset=ks[1], nstep_ref=s, **kw)
+    return pr, ppsd.manual_params(kstep=ks[0], koffset=ks[1], **kw)
+
+
+def _ref_unit(rng):
+    """A unit inside the reference's validation domain (derived window)."""
+    while True:
+        pmin = rng.randrange(2**40, 2**50)
+        kmax = rng.randrange(3, min(2**22, pmin * pmin >> 63))
+        kw = dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 40000), kmin=rng.randrange(1, kmax + 1), kmax=kmax,
+                  nmin=rng.randrange(64, 5000), sign=rng.choice((1, -1, 0)))
+        kw["nmax"] = kw["nmin"] + rng.randrange(0, 400000)
+        pr = orc.derive(**kw)
+        if pr.status == 0:
+            return pr, ppsd.derive(ppsd.SearchParams(**kw))
+
+
+def _oracle(pr, raw):
+    P = orc.primes(pr.pmin, pr.pmax)
+    hits = orc.hit_set(orc.walk(pr, P, filter=not raw, threads=8))
+    return len(P), int(P.sum(dtype=np.uint64)), hits
+
+
+def _per_unit(out, unit, n):
+    sets = [set() for _ in range(n)]
+    for f, u in zip(out, unit):
+        sets[int(u)].add((int(f["p"]), int(f["k"]), int(f["n"]), int(f["c"])))
+    return sets
+
+
+def _check_mix(units, ctx, raw):
+    out, unit, cnt, sums = ctx.batch_run([u[1] for u in units], raw=raw)
+    got = _per_unit(out, unit, len(units))
+    assert len(out) == sum(len(g) for g in got), "a hit was reported twice"
+    nhits = 0
+    for i, (pr, _) in enumerate(units):
+        c, s, want = _oracle(pr, raw)
+        assert (int(cnt[i]), int(sums[i])) == (c, s), f"unit {i}: prime count/sum"
+        assert got[i] == want, f"unit {i}: {len(got[i])} hits, oracle {len(want)}"
+        nhits += len(want)
+    return nhits
+
+
+# (r, u, waves per SIMD, seg_min, seg_max): the default shape, the other
+# kernel shapes, and tiny n-segments so that most tasks start mid-range
+SHAPES = [(1, 8, 8, 0, 0), (2, 4, 4, 16, 64), (1, 16, 2, 16, 16), (2, 16, 8, 64, 4096), (1, 4, 16, 32, 256),
+          (2, 8, 1, 2048, 65536)]
+
+
+@pytest.mark.parametrize("seed", range(len(SHAPES)))
+def test_batch_mix_vs_oracle(seed):
+    rng = random.Random(3000 + seed)
+    units = [_dense_unit(rng) if i % 3 else _ref_unit(rng) for i in range(12)]
+    with ppsd.Context(units[0][1]) as ctx:
+        ctx.set_batch_tuning(*SHAPES[seed])
+        assert _check_mix(units, ctx, raw=True) > 0
+        _check_mix(units, ctx, raw=False)
+
+
+def test_batch_golden_ranges_together():
+    """pps/many_n_test.sh's range, the 1e12 search (s = 27) and slices of the
+    2^50 and 2^61 (both signs) searches as four units of one batch: each
+    unit's factors and prime count/sum are its golden run's."""
+    tags = [("many_n", "many_n_filtered.txt"), ("cfg1", "cfg1_factors.txt"), ("cfg2_slice", "cfg2_slice.txt"),
+            ("cfg3_slice", "cfg3_slice.txt")]
+    params = [ppsd.derive(ppsd.SearchParams(**RUNS[t]["params"])) for t, _ in tags]
+    with ppsd.Context(params[0]) as ctx:
+        out, unit, cnt, sums = ctx.batch_run(params)
+        raw, raw_unit, _, _ = ctx.batch_run(params[:1], raw=True)
+        st = ctx.stats()
+    got = _per_unit(out, unit, len(tags))
+    for i, (t, name) in enumerate(tags):
+        assert got[i] == golden_set(name), t
+        assert (int(cnt[i]), int(sums[i])) == (RUNS[t]["count"], RUNS[t]["sum"]), t
+    assert _per_unit(raw, raw_unit, 1)[0] == golden_set("many_n_raw.txt")
+    assert st.primes == sum(RUNS[t]["count"] for t, _ in tags) + RUNS["many_n"]["count"]
+
+
+def test_batch_equals_plain_path():
+    """A unit's hits in a mix are the hits of searching it alone."""
+    rng = random.Random(4000)
+    units = [_ref_unit(rng) for _ in range(6)] + [_dense_unit(rng) for _ in range(6)]
+    with ppsd.Context(units[0][1]) as ctx:
+        out, unit, _, _ = ctx.batch_run([u[1] for u in units], raw=True)
+    got = _per_unit(out, unit, len(units))
+    for i, (pr, params) in enumerate(units):
+        with ppsd.Context(params) as ctx:
+            alone, _, _ = ctx.run_range(pr.pmin, pr.pmax, raw=True)
+            single, su, _, _ = ctx.batch_run([params], raw=True)
+        assert got[i] == ppsd.factor_set(alone), i
+        assert ppsd.factor_set(single) == got[i] and set(su.tolist()) <= {0}
+
+
+def test_batch_ring_overflow_reruns():
+    rng = random.Random(5000)
+    units = [_dense_unit(rng) for _ in range(8)]
+    with ppsd.Context(units[0][1]) as ctx:
+        ctx.set_batch_tuning(seg_min=16, seg_max=64)
+        full, fu, _, _ = ctx.batch_run([u[1] for u in units], raw=True)
+        ctx.set_batch_hit_capacity(512)
+        small, su, _, _ = ctx.batch_run([u[1] for u in units], raw=True)
+        assert ctx.stats().retries > 0
+    assert len(full) > 512
+    assert sorted(zip(map(tuple, full.tolist()), fu.tolist())) == sorted(zip(map(tuple, small.tolist()), su.tolist()))
+
+
+def test_batch_repeat_is_stable():
+    """Replays of the captured round give the same result every time."""
+    rng = random.Random(6000)
+    units = [_dense_unit(rng) for _ in range(5)]
+    with ppsd.Context(units[0][1]) as ctx:
+        first = ctx.batch_run([u[1] for u in units], raw=True)
+        for _ in range(3):
+            again = ctx.batch_run([u[1] for u in units], raw=True)
+            assert sorted(zip(map(tuple, again[0].tolist()), again[1].tolist())) == \
+                sorted(zip(map(tuple, first[0].tolist()), first[1].tolist()))
+            assert again[2].tolist() == first[2].tolist() and again[3].tolist() == first[3].tolist()
+
+
+def test_batch_empty_and_errors():
+    rng = random.Random(7000)
+    pr, params = _ref_unit(rng)
+    with ppsd.Context(params) as ctx:
+        out, unit, cnt, _ = ctx.batch_run([])
+        assert len(out) == 0 and len(cnt) == 0
+        # an empty n-range and an empty p-range
+        e1 = ppsd.manual_params(pr.pmin, pr.pmax, pr.kmin, pr.kmax, 100, 100, nstep=8)
+        e2 = ppsd.manual_params(pr.pmin, pr.pmin + 1, pr.kmin, pr.kmax, 100, 200, nstep=8)
+        out, unit, cnt, _ = ctx.batch_run([e1, e2])
+        assert len(out) == 0 and int(cnt[0]) > 0 and int(cnt[1]) <= 1
+        bad = ppsd.manual_params(pr.pmin, pr.pmax, pr.kmin, pr.kmax, 100, 200, nstep=8)
+        bad.kmin = bad.kmax + 1
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.batch_run([params, bad])
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.set_batch_tuning(r=3)
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+        ctx.submit_range(pr.pmin, pr.pmax)
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.batch_run([params])
+        assert e.value.code == ppsd.PPSD_ERR_STATE
+        ctx.collect()
+        ctx.batch_run([params])
