@@ -173,6 +173,23 @@ def test_dense_hit_configs_vs_oracle(seed):
         assert n > 0
 
 
+@pytest.mark.parametrize("sign", (1, -1))
+def test_filter_high_k_vs_oracle(sign):
+    """k up to 2^62 (s = 1, p just below 2^62): the khigh guards of
+    test_factor and its wrapping 64-bit pre-tests (pps/app.c:171-179) on the
+    device, against the oracle's restatement of them."""
+    pmin, pmax = 2**62 - 20000, 2**62
+    kw = dict(pmin=pmin, pmax=pmax, kmin=2**46, kmax=pmin - 1, nmin=64, nmax=1000, sign=sign, nstep=1)
+    pr = orc.OrcParams(kstep=2, koffset=1, nstep_ref=1, **kw)
+    P = orc.primes(pmin, pmax)
+    want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
+    with ppsd.Context(ppsd.manual_params(**kw)) as ctx:
+        got, cnt, _ = ctx.run_range(pmin, pmax)
+    assert cnt == len(P)
+    assert ppsd.factor_set(got) == want
+    assert sum(1 for t in want if t[1] >= 2**61) > 100
+
+
 def test_empty_inputs():
     # nmin == nmax: no exponent in range (the reference rejects it: nstep 1)
     params = ppsd.manual_params(MANY_N["pmin"], MANY_N["pmax"], 1201, 9999, 76, 76, nstep=32)

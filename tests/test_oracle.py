@@ -229,6 +229,20 @@ def test_test_factor_rules():
         c = rng.choice((1, -1))
         want = (k & 1) and 100 <= n < 1000 and smallest_factor(k, n, c) == 0
         assert orc.lib().orc_test_factor(pr, k, n, c) == int(bool(want)), (k, n, c)
+    # large k: the khigh guards (pps/app.c:171-179) skip pre-tests from
+    # k >= 2^46 on, which try_all_factors then covers; below 2^61 no shift
+    # of the pre-tests wraps, so the kept set is exactly "no factor < 32768"
+    pr = orc.OrcParams(pmin=2**62 - 2**40, pmax=2**62, kmin=1, kmax=2**62 - 1, nmin=64, nmax=2**31, kstep=2,
+                       koffset=1, sign=1, nstep=1, nstep_ref=1)
+    edges = [2**46, 2**52, 2**53, 2**54, 2**56, 2**61]
+    for _ in range(600):
+        e = rng.choice(edges)
+        k = (rng.randrange(e - 2**20, e + 2**20) if rng.random() < 0.5 else rng.randrange(2**46, 2**61)) | 1
+        k = min(k, 2**61 - 1)
+        n = rng.randrange(64, 2**24)
+        c = rng.choice((1, -1))
+        want = smallest_factor(k, n, c) == 0
+        assert orc.lib().orc_test_factor(pr, k, n, c) == int(want), (k, n, c)
     # -M 6 -s 3: only k == 3 (mod 6)
     pr = orc.derive(pmin=10**12, kmin=3, kmax=9999, nmin=100, nmax=1000, kstep=6, koffset=3)
     for k in range(3, 400, 2):
