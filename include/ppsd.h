@@ -124,7 +124,7 @@ int ppsd_collect_raw(ppsd_ctx *ctx, ppsd_factor *out, uint64_t cap, uint64_t *n_
 int ppsd_set_tuning(ppsd_ctx *ctx, int r, int u, int wg);
 int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
 
-/* Raw-hit ring capacity for later submissions (1 .. the allocated 2^22).
+/* Raw-hit ring capacity for later submissions (1 .. the allocated 2^24).
  * A batch whose raw hits overflow the ring is re-walked in halves, so the
  * capacity only trades memory for re-walks; the result never changes. */
 int ppsd_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
@@ -172,9 +172,63 @@ int ppsd_batch_set_tuning(ppsd_ctx *ctx, int r, int u, int waves_per_simd, uint3
 int ppsd_batch_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *waves_per_simd, uint32_t *seg_min,
                           uint32_t *seg_max);
 
-/* Raw-hit ring capacity of the batched walk (1 .. 2^22); overflowing task
+/* Raw-hit ring capacity of the batched walk (1 .. 2^25); overflowing task
  * sets are re-run in halves, so results never depend on it. */
 int ppsd_batch_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
+
+/* ---- Serving loop (no counterpart in the reference).  For callers that
+ * receive searches continuously (a work-unit server) and need each answered
+ * soon after it arrives.  One server owns one device and a worker thread.
+ *
+ * Requests are full ppsd_params, as for ppsd_batch_run.  They queue in
+ * arrival order and are admitted while the device memory they hold -- their
+ * share of the resident prime pool (primes of their next p-piece, at most a
+ * quarter of the pool) plus a hit-ring reservation -- fits mem_budget.
+ * The device then runs slices of about slice_ms: each slice is one round of
+ * the batched walk over one p-piece of every admitted request, the slice's
+ * work shared equally among them, smallest remaining work first.  A request
+ * arriving during a slice joins the next one.  Each request's result equals
+ * searching it alone (ppsd_submit_range): same factors (or raw divisors),
+ * same prime count and sum.  Times are seconds since ppsd_serve_open. */
+typedef struct ppsd_server ppsd_server;
+
+typedef struct ppsd_serve_config {
+  uint64_t mem_budget; /* bytes admitted requests may hold (0 = pool + rings, about 0.8 GiB) */
+  double slice_ms;     /* target length of one slice (0 = 50) */
+  uint32_t max_active; /* admitted requests at most (0 = 4096; <= 65536) */
+  int32_t raw;         /* 1: raw divisors instead of filtered factors */
+} ppsd_serve_config;
+
+typedef struct ppsd_serve_result {
+  uint64_t id;
+  int32_t status; /* 0, or the error code that ended the request */
+  uint32_t slices;
+  uint64_t primes, prime_sum, factors, window_steps;
+  double t_arrival, t_admit, t_start, t_done; /* queue = admit - arrival .. start; service = done - start */
+} ppsd_serve_result;
+
+typedef struct ppsd_serve_stats {
+  uint64_t submitted, admitted, completed, queued, active;
+  uint64_t slices, primes, window_steps;
+  uint64_t mem_in_use, mem_peak, mem_budget;
+  double busy_s; /* time spent in slices */
+  double now;
+} ppsd_serve_stats;
+
+/* Opens device `device` (cfg may be NULL) and starts the worker. */
+int ppsd_serve_open(int device, const ppsd_serve_config *cfg, ppsd_server **out);
+/* Queues a request (thread-safe); *id identifies its result. */
+int ppsd_serve_submit(ppsd_server *s, const ppsd_params *req, uint64_t *id);
+/* Takes the next completed request, waiting up to timeout_ms: 0 = a result
+ * in *res, 1 = none within the timeout. */
+int ppsd_serve_next(ppsd_server *s, int timeout_ms, ppsd_serve_result *res);
+/* Factors of a taken result: up to cap to out, *n_out = their number. */
+int ppsd_serve_factors(ppsd_server *s, uint64_t id, ppsd_factor *out, uint64_t cap, uint64_t *n_out);
+/* Frees a taken result. */
+int ppsd_serve_release(ppsd_server *s, uint64_t id);
+int ppsd_serve_get_stats(ppsd_server *s, ppsd_serve_stats *st);
+/* Stops the worker after its current slice; unfinished requests are dropped. */
+int ppsd_serve_close(ppsd_server *s);
 
 /* Arithmetic check (no counterpart in the reference): out[i] = 2^-e mod P[i]
  * after nwin further windows of s, i.e. 2^-(e + nwin*s) mod P[i], computed by

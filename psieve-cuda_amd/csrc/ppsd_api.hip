@@ -1,25 +1,13 @@
 // ppsd_api.hip -- host half of libppsd.so: the C ABI declared in
 // include/ppsd.h.  Owns every device buffer of a context; launches the
 // kernels of ppsd_kernels.hip on one non-blocking stream per context.
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdarg.h>
-#include <stdio.h>
-#include <string.h>
-
-#include <algorithm>
-#include <string>
-#include <vector>
-
-#include "../../include/ppsd.h"
-#include "ppsd_batch.h"
-#include "ppsd_kernels.h"
+#include "ppsd_ctx.h"
 
 using namespace ppsd;
 
-namespace {
+static thread_local std::string g_err;
 
-thread_local std::string g_err;
+namespace ppsd {
 
 int set_err(int code, const char *fmt, ...) {
   char buf[512];
@@ -31,22 +19,10 @@ int set_err(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                               \
-  do {                                                                                              \
-    hipError_t e_ = (expr);                                                                         \
-    if (e_ != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error: %s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
 uint64_t isqrt64(uint64_t x) {
   uint64_t r = (uint64_t)sqrt((double)x);
   while (r * r > x) --r;
   while ((r + 1) * (r + 1) <= x) ++r;
-  return r;
-}
-
-int lg2(uint64_t v) {
-  int r = 0;
-  while (v >>= 1) ++r;
   return r;
 }
 
@@ -62,76 +38,15 @@ std::vector<uint32_t> host_primes(uint32_t lim) {
   return out;
 }
 
-enum class Pending { none, primes, range };
+}  // namespace ppsd
 
-}  // namespace
+static int lg2(uint64_t v) {
+  int r = 0;
+  while (v >>= 1) ++r;
+  return r;
+}
 
-struct ppsd_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  ppsd_params prm{};
-  uint32_t batch = 0;
-  int tune_r = 1, tune_u = 8, tune_wg = 256;
 
-  // device buffers
-  uint64_t *d_P = nullptr;
-  uint32_t *d_count = nullptr;             // primes in d_P (range mode)
-  unsigned long long *d_counters = nullptr;  // [0] raw hits, [1] kept, [2] prime sum
-  Hit *d_hits = nullptr;
-  Hit *d_kept = nullptr;
-  uint64_t hit_cap = 0;        // capacity in use
-  uint64_t hit_alloc = 0;      // allocated entries
-  uint32_t *d_small = nullptr;  // odd primes 11 <= q < 32768
-  uint32_t nsmall = 0;
-  uint32_t *d_sp = nullptr;     // sieving primes <= qmax
-  uint32_t nsp = 0, nsp_small = 0;
-  uint64_t qmax = 0;
-  std::vector<uint32_t> sp_sample;  // d_sp[0], d_sp[4096], ... (batched walk)
-  uint32_t *d_bitmap = nullptr;
-  uint32_t *d_blockcount = nullptr;
-  uint64_t nbits_max = 0;
-  uint64_t *h_stage = nullptr;  // pinned upload staging
-
-  hipEvent_t ev[6] = {};  // walk begin/end, sieve begin/end, filter begin/end
-
-  // outstanding submission
-  Pending pending = Pending::none;
-  uint32_t n_primes_host = 0;  // prime count when known on the host
-  bool sieved = false;
-  uint64_t range_lo = 0, range_nbits = 0;  // candidates of a range submission
-  std::vector<Hit> result;
-  bool result_raw = false;
-  uint64_t last_primes = 0, last_sum = 0;
-  bool have_result = false;
-
-  ppsd_stats st{};
-
-  // batched walk (allocated on first use)
-  bool b_ready = false;
-  uint64_t b_pool_cap = 0;
-  uint64_t *b_pool = nullptr;
-  uint32_t *b_pool_count = nullptr;
-  unsigned long long *b_pool_sum = nullptr;
-  uint32_t b_piece_cap = 0, b_task_cap = 0;
-  PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
-  Task *b_tasks = nullptr, *h_tasks = nullptr;
-  uint32_t *b_ntasks = nullptr, *b_next = nullptr, *h_ntasks = nullptr;
-  BHit *b_hits = nullptr, *b_kept = nullptr;
-  uint64_t b_hit_cap = 0, b_hit_alloc = 0;
-  unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
-  uint32_t *b_off = nullptr;
-  unsigned long long *b_sums = nullptr;
-  hipGraph_t b_graph = nullptr;
-  hipGraphExec_t b_exec = nullptr;
-  int b_graph_key = -1;
-  int b_r = 1, b_u = 8, b_waves = 8;
-  uint32_t b_seg_min = 2048, b_seg_max = 65536;
-  std::vector<BHit> b_result;
-  std::vector<uint64_t> b_unit_primes, b_unit_sums;
-  uint32_t b_nunits = 0;
-};
-
-extern "C" {
 
 int ppsd_derive_params(ppsd_params *pr) {
   const uint64_t PMAX_MAX = (uint64_t)1 << 62;
@@ -195,7 +110,7 @@ const char *ppsd_last_error(void) { return g_err.c_str(); }
 // What the walk itself needs (ppsd_derive_params gives a subset of this):
 // one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
 // nmin >= 64 for the 2^-nmin set-up.
-static bool walk_domain_ok(const ppsd_params *p) {
+bool ppsd::walk_domain_ok(const ppsd_params *p) {
   return !(p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) ||
            p->pmax > ((uint64_t)1 << 62) || p->pmin >= p->pmax || p->nmin < 64 || p->nmin > p->nmax ||
            p->kmin < 1 || p->kmin > p->kmax || (p->kmax >> (64 - p->nstep)) != 0 ||
@@ -296,7 +211,7 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   c->device = device;
   c->prm = *p;
   c->batch = batch ? batch : (1u << 21);
-  c->hit_cap = c->hit_alloc = 1u << 22;
+  c->hit_cap = c->hit_alloc = 1u << 24;
   c->nbits_max = (uint64_t)1 << 27;
   int rc = 0;
   auto bail = [&](int code) {
@@ -384,7 +299,7 @@ int ppsd_reset_stats(ppsd_ctx *c) {
   return 0;
 }
 
-static uint32_t windows(const ppsd_params &p) {
+uint32_t ppsd::windows(const ppsd_params &p) {
   return p.nmax > p.nmin ? (p.nmax - p.nmin + p.nstep - 1) / p.nstep : 0;
 }
 
@@ -426,7 +341,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.kept_count = c->d_counters + 1;
   f.kept_cap = c->hit_cap;
   HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  HIP_TRY(launch_filter(f, 1024, c->stream));
+  HIP_TRY(launch_filter(f, 4096, c->stream));
   HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   c->st.walk_launches++;
   return 0;
@@ -459,17 +374,25 @@ int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
   return 0;
 }
 
+double ppsd::hits_per_prime(const ppsd_params &p, uint64_t lo) {
+  const double kodd = (double)(p.kmax - p.kmin) / 2 + 1;
+  return (double)windows(p) * p.nstep * kodd / (double)lo * (p.sign == 0 ? 2 : 1);
+}
+
 // Odd candidates from lo whose expected prime count (2/ln x of them),
-// plus six standard deviations, fits the batch.  A rarer excess is handled
-// in finish() by sieving the chunk again in halves.
+// plus six standard deviations, fits the batch, and whose expected raw hits
+// fill at most half the ring.  A rarer excess is handled in finish() by
+// sieving the chunk again in halves (primes) or walking it in halves (hits).
 static uint64_t range_fit(const ppsd_ctx *c, uint64_t lo) {
   const double dens = 2.0 / log((double)lo);
   const double room = (double)c->batch - 32.0;
   if (room <= 1.0) return 1;
   const double r = (-6.0 + sqrt(36.0 + 4.0 * room)) / 2.0;  // mu + 6 sqrt(mu) = room
-  uint64_t fit = (uint64_t)(r * r / dens);
-  fit = std::min<uint64_t>(fit, c->nbits_max);
-  return fit ? fit : 1;
+  double fit = r * r / dens;
+  const double hpp = hits_per_prime(c->prm, lo);
+  if (hpp > 0) fit = std::min(fit, 0.5 * (double)c->hit_cap / hpp / dens);
+  fit = std::min(fit, (double)c->nbits_max);
+  return fit >= 1 ? (uint64_t)fit : 1;
 }
 
 // Sieve odd candidates [lo, lo + 2 nbits) into d_P (count in d_count).
@@ -516,7 +439,7 @@ int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
   return 0;
 }
 
-static float elapsed(hipEvent_t a, hipEvent_t b) {
+float ppsd::elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
   return ms;
@@ -685,12 +608,12 @@ int ppsd_close(ppsd_ctx *c) { return free_ctx(c); }
 // batched walk: many work units per launch
 // ---------------------------------------------------------------------------
 
-static int batch_alloc(ppsd_ctx *c) {
+int ppsd::batch_alloc(ppsd_ctx *c) {
   if (c->b_ready) return 0;
   c->b_pool_cap = (uint64_t)1 << 26;
   c->b_piece_cap = 1u << 16;
   c->b_task_cap = 1u << 22;
-  c->b_hit_cap = c->b_hit_alloc = (uint64_t)1 << 22;
+  c->b_hit_cap = c->b_hit_alloc = (uint64_t)1 << 25;
   HIP_TRY(hipMalloc(&c->b_pool, c->b_pool_cap * sizeof(uint64_t)));
   HIP_TRY(hipMalloc(&c->b_pool_count, sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&c->b_pool_sum, sizeof(unsigned long long)));
@@ -714,7 +637,7 @@ static int batch_alloc(ppsd_ctx *c) {
 // host sample of every kSpSample-th of them to bound each piece's share.
 constexpr uint32_t kSpSample = 4096;
 
-static int ensure_sieving(ppsd_ctx *c, uint64_t pmax) {
+int ppsd::ensure_sieving(ppsd_ctx *c, uint64_t pmax) {
   const uint64_t need = isqrt64(pmax - 1);
   if (need > c->qmax) {
     (void)hipFree(c->d_sp);
@@ -784,7 +707,7 @@ static int round_graph(ppsd_ctx *c) {
   hipError_t e = launch_zero_u64(c->b_counts, 2, c->stream);
   if (e == hipSuccess) e = launch_zero_u32(c->b_next, 1, c->stream);
   if (e == hipSuccess) e = launch_persist(a, c->b_r, c->b_u, grid, c->stream);
-  if (e == hipSuccess) e = launch_batch_filter(a, 1024, c->stream);
+  if (e == hipSuccess) e = launch_batch_filter(a, 4096, c->stream);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (e != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error capturing the round: %s", hipGetErrorString(e));
@@ -795,11 +718,6 @@ static int round_graph(ppsd_ctx *c) {
   return 0;
 }
 
-struct Piece {
-  uint32_t unit;
-  uint64_t lo, hi;  // [lo, hi)
-  double est;       // expected primes
-};
 
 // Tasks of one round over the sieved pieces; the segment length halves from
 // b_seg_max down to b_seg_min until the queue holds enough tasks to keep
@@ -878,7 +796,7 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw) {
 
 // One round: sieve the pieces into the pool, build and drain the queue.
 // Returns 1 (no error) when the pool overflowed and the caller must split.
-static int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
                      const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit) {
   const uint32_t np = (uint32_t)(e - b);
   HIP_TRY(launch_zero_u32(c->b_pool_count, 1, c->stream));
@@ -939,11 +857,27 @@ static int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
   HIP_TRY(hipMemcpyAsync(c->b_pieces, c->h_pieces, np * sizeof(PieceDesc), hipMemcpyHostToDevice, c->stream));
   std::vector<Task> tasks;
   build_tasks(c, par, np, tasks);
+  // task groups whose expected raw hits fill at most half the ring
+  std::vector<double> hpw(np);  // expected hits per prime-window of each piece
+  for (uint32_t i = 0; i < np; ++i) {
+    const PieceDesc &D = c->h_pieces[i];
+    hpw[i] = D.nwin ? hits_per_prime(*par[pieces[b + i].unit], pieces[b + i].lo | 1) / D.nwin : 0.0;
+  }
+  const double room = 0.5 * (double)c->b_hit_cap;
   const size_t first_hit = c->b_result.size();
-  for (size_t t = 0; t < tasks.size(); t += c->b_task_cap) {
-    const uint32_t k = (uint32_t)std::min<size_t>(c->b_task_cap, tasks.size() - t);
-    int rc = run_tasks(c, tasks.data() + t, k, raw);
+  for (size_t t = 0; t < tasks.size();) {
+    size_t k = 0;
+    double e = 0;
+    while (t + k < tasks.size() && k < c->b_task_cap) {
+      const Task &T = tasks[t + k];
+      const double et = hpw[T.piece] * T.count * T.nw;
+      if (k && e + et > room) break;
+      e += et;
+      ++k;
+    }
+    int rc = run_tasks(c, tasks.data() + t, (uint32_t)k, raw);
     if (rc) return rc;
+    t += k;
   }
   // piece index -> caller's unit
   for (size_t h = first_hit; h < c->b_result.size(); ++h) c->b_result[h].piece = piece_unit[c->b_result[h].piece];
@@ -1091,4 +1025,3 @@ int ppsd_debug_state(int device, const uint64_t *P, uint32_t n, uint32_t e, uint
   return 0;
 }
 
-}  // extern "C"

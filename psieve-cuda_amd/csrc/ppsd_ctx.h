@@ -1,0 +1,134 @@
+// ppsd_ctx.h -- the context behind the C ABI, shared by the host-side files
+// of libppsd.so (ppsd_api.hip, ppsd_serve.hip).  Not installed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ppsd.h"
+#include "ppsd_batch.h"
+#include "ppsd_kernels.h"
+
+namespace ppsd {
+
+// Records the message for ppsd_last_error (per host thread); returns code.
+int set_err(int code, const char *fmt, ...);
+uint64_t isqrt64(uint64_t x);
+std::vector<uint32_t> host_primes(uint32_t lim);
+
+enum class Pending { none, primes, range };
+
+// A p-subrange of one unit of a batched search.
+struct Piece {
+  uint32_t unit;
+  uint64_t lo, hi;  // [lo, hi)
+  double est;       // expected primes
+};
+
+}  // namespace ppsd
+
+#define HIP_TRY(expr)                                                                               \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      return ppsd::set_err(PPSD_ERR_FAILURE, "HIP error: %s: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+struct ppsd_ctx {
+  using Hit = ppsd::Hit;
+  using PieceDesc = ppsd::PieceDesc;
+  using Task = ppsd::Task;
+  using BHit = ppsd::BHit;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ppsd_params prm{};
+  uint32_t batch = 0;
+  int tune_r = 1, tune_u = 8, tune_wg = 256;
+
+  // device buffers
+  uint64_t *d_P = nullptr;
+  uint32_t *d_count = nullptr;             // primes in d_P (range mode)
+  unsigned long long *d_counters = nullptr;  // [0] raw hits, [1] kept, [2] prime sum
+  Hit *d_hits = nullptr;
+  Hit *d_kept = nullptr;
+  uint64_t hit_cap = 0;        // capacity in use
+  uint64_t hit_alloc = 0;      // allocated entries
+  uint32_t *d_small = nullptr;  // odd primes 11 <= q < 32768
+  uint32_t nsmall = 0;
+  uint32_t *d_sp = nullptr;     // sieving primes <= qmax
+  uint32_t nsp = 0, nsp_small = 0;
+  uint64_t qmax = 0;
+  std::vector<uint32_t> sp_sample;  // d_sp[0], d_sp[4096], ... (batched walk)
+  uint32_t *d_bitmap = nullptr;
+  uint32_t *d_blockcount = nullptr;
+  uint64_t nbits_max = 0;
+  uint64_t *h_stage = nullptr;  // pinned upload staging
+
+  hipEvent_t ev[6] = {};  // walk begin/end, sieve begin/end, filter begin/end
+
+  // outstanding submission
+  ppsd::Pending pending = ppsd::Pending::none;
+  uint32_t n_primes_host = 0;  // prime count when known on the host
+  bool sieved = false;
+  uint64_t range_lo = 0, range_nbits = 0;  // candidates of a range submission
+  std::vector<Hit> result;
+  bool result_raw = false;
+  uint64_t last_primes = 0, last_sum = 0;
+  bool have_result = false;
+
+  ppsd_stats st{};
+
+  // batched walk (allocated on first use)
+  bool b_ready = false;
+  uint64_t b_pool_cap = 0;
+  uint64_t *b_pool = nullptr;
+  uint32_t *b_pool_count = nullptr;
+  unsigned long long *b_pool_sum = nullptr;
+  uint32_t b_piece_cap = 0, b_task_cap = 0;
+  PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
+  Task *b_tasks = nullptr, *h_tasks = nullptr;
+  uint32_t *b_ntasks = nullptr, *b_next = nullptr, *h_ntasks = nullptr;
+  BHit *b_hits = nullptr, *b_kept = nullptr;
+  uint64_t b_hit_cap = 0, b_hit_alloc = 0;
+  unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
+  uint32_t *b_off = nullptr;
+  unsigned long long *b_sums = nullptr;
+  hipGraph_t b_graph = nullptr;
+  hipGraphExec_t b_exec = nullptr;
+  int b_graph_key = -1;
+  int b_r = 1, b_u = 8, b_waves = 8;
+  uint32_t b_seg_min = 2048, b_seg_max = 65536;
+  std::vector<BHit> b_result;
+  std::vector<uint64_t> b_unit_primes, b_unit_sums;
+  uint32_t b_nunits = 0;
+};
+
+namespace ppsd {
+
+// What the walk needs of a unit's parameters.
+bool walk_domain_ok(const ppsd_params *p);
+// Windows of s over [nmin, nmax).
+uint32_t windows(const ppsd_params &p);
+// Expected raw hits of one prime near lo: every window holds about
+// s*(kmax-kmin)/2 candidate values out of p (both signs: twice that).
+double hits_per_prime(const ppsd_params &p, uint64_t lo);
+// Milliseconds between two recorded events (0 on error).
+float elapsed(hipEvent_t a, hipEvent_t b);
+// Batched walk: buffers (allocated on first use), sieving primes for every
+// p < pmax, and one round over pieces [b, e) of a piece list.  run_round
+// appends the round's hits to c->b_result tagged with piece_unit[] of their
+// piece, adds to c->b_unit_primes / b_unit_sums, and returns 1 (no error)
+// when the pieces overflowed the prime pool (nothing recorded then).
+int batch_alloc(ppsd_ctx *c);
+int ensure_sieving(ppsd_ctx *c, uint64_t pmax);
+int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+              const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit);
+
+}  // namespace ppsd

@@ -73,9 +73,13 @@ __device__ __forceinline__ bool keep_factor(uint64_t k, uint32_t n, int32_t c, u
   uint64_t mod31 = 1;
   if (keep) keep = pre_rules(k, n, c, mod31) && (uint32_t)(mod31 % 31) != 0;
   if (keep) {
-    bool d = false;
-    for (uint32_t j = __lane_id(); j < nsmall; j += 64) d |= divides(k, n, c, small_primes[j]);
-    keep = __ballot(d) == 0;
+    // 64 primes per pass, smallest first; most composites fall in the first
+    // passes, so the wave stops at the first pass that finds a divisor
+    for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
+      const uint32_t j = j0 + __lane_id();
+      const bool d = j < nsmall && divides(k, n, c, small_primes[j]);
+      if (__ballot(d) != 0) return false;
+    }
   }
   return keep;
 }

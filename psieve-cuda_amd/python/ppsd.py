@@ -53,6 +53,23 @@ class Stats(C.Structure):
 
 FACTOR_DTYPE = np.dtype([("p", "<u8"), ("k", "<u8"), ("n", "<u4"), ("c", "<i4")])
 
+class ServeConfig(C.Structure):
+    _fields_ = [("mem_budget", C.c_uint64), ("slice_ms", C.c_double), ("max_active", C.c_uint32),
+                ("raw", C.c_int32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("slices", C.c_uint32), ("primes", C.c_uint64),
+                ("prime_sum", C.c_uint64), ("factors", C.c_uint64), ("window_steps", C.c_uint64),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "admitted", "completed", "queued", "active", "slices",
+                                          "primes", "window_steps", "mem_in_use", "mem_peak", "mem_budget")] + \
+               [("busy_s", C.c_double), ("now", C.c_double)]
+
+
 _P = C.POINTER
 _u64p = _P(C.c_uint64)
 
@@ -85,6 +102,13 @@ def _load():
         "ppsd_batch_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int),
                                             _P(C.c_uint32), _P(C.c_uint32)]),
         "ppsd_batch_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "ppsd_serve_open": (C.c_int, [C.c_int, _P(ServeConfig), _P(C.c_void_p)]),
+        "ppsd_serve_submit": (C.c_int, [C.c_void_p, _P(Params), _u64p]),
+        "ppsd_serve_next": (C.c_int, [C.c_void_p, C.c_int, _P(ServeResult)]),
+        "ppsd_serve_factors": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, _u64p]),
+        "ppsd_serve_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "ppsd_serve_get_stats": (C.c_int, [C.c_void_p, _P(ServeStats)]),
+        "ppsd_serve_close": (C.c_int, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -285,6 +309,58 @@ class Context:
 
     def set_batch_hit_capacity(self, cap: int):
         _check(lib().ppsd_batch_set_hit_capacity(self._h, cap), "ppsd_batch_set_hit_capacity")
+
+
+class Server:
+    """Serving loop on one device (ppsd_serve_*): submit searches at any
+    time, take completed ones with their timings and factors."""
+
+    def __init__(self, device: int = 0, mem_budget: int = 0, slice_ms: float = 0.0, max_active: int = 0,
+                 raw: bool = False):
+        self._h = C.c_void_p()
+        cfg = ServeConfig(mem_budget=mem_budget, slice_ms=slice_ms, max_active=max_active, raw=1 if raw else 0)
+        _check(lib().ppsd_serve_open(device, C.byref(cfg), C.byref(self._h)), "ppsd_serve_open")
+
+    def submit(self, params: Params) -> int:
+        rid = C.c_uint64()
+        _check(lib().ppsd_serve_submit(self._h, C.byref(params), C.byref(rid)), "ppsd_serve_submit")
+        return rid.value
+
+    def next(self, timeout_ms: int = 100):
+        """(ServeResult, factors) of the next completed request, or None."""
+        res = ServeResult()
+        rc = lib().ppsd_serve_next(self._h, timeout_ms, C.byref(res))
+        if rc == 1:
+            return None
+        _check(rc, "ppsd_serve_next")
+        n = C.c_uint64()
+        out = np.zeros(res.factors, dtype=FACTOR_DTYPE)
+        _check(lib().ppsd_serve_factors(self._h, res.id, out.ctypes.data_as(C.c_void_p), len(out), C.byref(n)),
+               "ppsd_serve_factors")
+        _check(lib().ppsd_serve_release(self._h, res.id), "ppsd_serve_release")
+        return res, out
+
+    def stats(self) -> ServeStats:
+        st = ServeStats()
+        _check(lib().ppsd_serve_get_stats(self._h, C.byref(st)), "ppsd_serve_get_stats")
+        return st
+
+    def close(self):
+        if self._h:
+            lib().ppsd_serve_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def debug_state(P, e: int, nwin: int, s: int, device: int = 0) -> np.ndarray:

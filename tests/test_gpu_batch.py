@@ -11,6 +11,7 @@ import pytest
 import pps_oracle as orc
 import ppsd
 from conftest import GOLDEN, read_lines
+from unitgen import _dense_unit, _oracle, _ref_unit
 
 pytestmark = pytest.mark.gpu
 
@@ -20,38 +21,6 @@ U64 = 2**64 - 1
 
 def golden_set(name):
     return {orc.parse_line(l) for l in read_lines(name)}
-
-
-def _dense_unit(rng):
-    """A unit with many hits per prime (small windows, large kmax)."""
-    pmin = rng.randrange(2**32 + 1, 2**36)
-    s = rng.randrange(1, 15) if rng.random() < 0.85 else rng.choice((31, 32))
-    kmax = max(1, min((pmin - 1) >> (s - 1), rng.randrange(2**12, 2**26)))
-    kw = dict(pmin=pmin, pmax=pmin + rng.randrange(500, 20000), kmin=rng.randrange(1, kmax + 1), kmax=kmax,
-              nmin=rng.randrange(64, 3000), sign=rng.choice((1, -1, 0)), nstep=s)
-    kw["nmax"] = kw["nmin"] + rng.randrange(0, 20000)
-    ks = rng.choice([(2, 1), (2, 1), (6, 3), (4, 1)])
-    pr = orc.OrcParams(kstep=ks[0], koffset=ks[1], nstep_ref=s, **kw)
-    return pr, ppsd.manual_params(kstep=ks[0], koffset=ks[1], **kw)
-
-
-def _ref_unit(rng):
-    """A unit inside the reference's validation domain (derived window)."""
-    while True:
-        pmin = rng.randrange(2**40, 2**50)
-        kmax = rng.randrange(3, min(2**22, pmin * pmin >> 63))
-        kw = dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 40000), kmin=rng.randrange(1, kmax + 1), kmax=kmax,
-                  nmin=rng.randrange(64, 5000), sign=rng.choice((1, -1, 0)))
-        kw["nmax"] = kw["nmin"] + rng.randrange(0, 400000)
-        pr = orc.derive(**kw)
-        if pr.status == 0:
-            return pr, ppsd.derive(ppsd.SearchParams(**kw))
-
-
-def _oracle(pr, raw):
-    P = orc.primes(pr.pmin, pr.pmax)
-    hits = orc.hit_set(orc.walk(pr, P, filter=not raw, threads=8))
-    return len(P), int(P.sum(dtype=np.uint64)), hits
 
 
 def _per_unit(out, unit, n):
@@ -130,9 +99,13 @@ def test_batch_ring_overflow_reruns():
     with ppsd.Context(units[0][1]) as ctx:
         ctx.set_batch_tuning(seg_min=16, seg_max=64)
         full, fu, _, _ = ctx.batch_run([u[1] for u in units], raw=True)
+        launches = ctx.stats().walk_launches
         ctx.set_batch_hit_capacity(512)
         small, su, _, _ = ctx.batch_run([u[1] for u in units], raw=True)
-        assert ctx.stats().retries > 0
+        # the task set ran in many groups sized for the ring, re-run in
+        # halves where a group still overflowed
+        st = ctx.stats()
+        assert st.walk_launches - launches > 2 * launches
     assert len(full) > 512
     assert sorted(zip(map(tuple, full.tolist()), fu.tolist())) == sorted(zip(map(tuple, small.tolist()), su.tolist()))
 

@@ -79,7 +79,7 @@ def test_many_n_host_primes():
 
 def test_hit_ring_overflow_rewalks():
     with context(**MANY_N) as ctx:
-        ctx.set_hit_capacity(7)
+        ctx.set_hit_capacity(1)
         raw, _, _ = ctx.run_range(MANY_N["pmin"], MANY_N["pmax"], raw=True)
         assert ppsd.factor_set(raw) == golden_set("many_n_raw.txt")
         assert ctx.stats().retries > 0
