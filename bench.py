@@ -27,15 +27,12 @@ METRIC = "primes/sec sieved (k≤1e4, n≤6e6) batched vs plain; % of INT32 VALU
 WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000, sign=1)
 
 # INT32 VALU peak of one MI355X: 256 CUs x 128 lanes x 2.4 GHz = 78.6 TOP/s,
-# i.e. the published 157.3 TFLOP/s FP32 vector peak / 2 flops per FMA.  Simple
-# 32-bit ops (v_add_u32, v_xor_b32, v_mov_b32) issue at that rate on gfx950.
+# i.e. the published 157.3 TFLOP/s FP32 vector peak / 2 flops per FMA.
 PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
-# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32 lane-ops,
-# a MUL32 weighted by R_MUL ALU32 slots: multiplies (v_mul_lo_u32,
-# v_mul_hi_u32, v_mad_u64_u32) issue at half the simple-op rate (measured).
-MUL32_PER_STEP = 4
-ALU32_PER_STEP = 11
-R_MUL = 2.0
+# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32 lane-ops
+# (the step's multiplies, carries and reduction, the ctz + shift + compare
+# candidate test), counted as 15 32-bit ops against that peak.
+OPS_PER_STEP = 4 + 11
 # HBM bytes one walk launch moves per prime walked, from the FETCH_SIZE and
 # WRITE_SIZE counters (scaled by a 256 MiB copy kernel's readings) on a
 # 2,088,811-prime launch: 16.0 MiB fetched + 4.6 KiB written.
@@ -51,6 +48,8 @@ def parse():
     ap.add_argument("--tune", default="", help="r,u,wg kernel shape")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mix-units", type=int, default=256,
+                    help="units of the seeded mixed workload run batched and plain (0 = skip)")
     return ap.parse_args()
 
 
@@ -74,6 +73,48 @@ def cpu_baseline(target_s):
     return {"value": m / dt, "unit": "primes/s", "cores": threads, "kind": "port",
             "sample": f"first {m} primes >= 2^50 of the workload, full n-range [50000,6000000), "
                       f"{dt:.1f} s on {threads} threads"}
+
+
+def mixed_units(n):
+    """The first n units of the seeded mixed workload (4096 units, one
+    persistent launch) searched twice: batched (ppsd_batch_run) and one unit
+    at a time on the plain path (contexts opened outside the clock).  Both
+    must give the same factors, prime counts and sums per unit."""
+    import numpy as np
+
+    import ppsd
+    import ppsd_units
+
+    units = ppsd_units.mix_units(n)
+    params = [u.params for u in units]
+    ctx = ppsd.Context(params[0])
+    ctx.batch_run(params[:4])  # first-use allocation and graph capture
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    out, unit, cnt, sums = ctx.batch_run(params)
+    tb = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    tp, parts, same_counts = 0.0, [], True
+    for i, p in enumerate(params):
+        with ppsd.Context(p) as c:
+            t0 = time.perf_counter()
+            f, cn, ps = c.run_range(p.pmin, p.pmax)
+            tp += time.perf_counter() - t0
+        same_counts &= (cn, ps) == (int(cnt[i]), int(sums[i]))
+        parts.append((np.full(len(f), i, np.uint32), f))
+
+    def canon(u, f):  # rows (unit, p, k, n, c) in lexicographic order
+        order = np.lexsort((f["c"], f["n"], f["k"], f["p"], u))
+        return u[order], f[order]
+
+    pu, pf = canon(np.concatenate([u for u, _ in parts]), np.concatenate([f for _, f in parts]))
+    bu, bf = canon(unit, out)
+    identical = same_counts and len(pu) == len(bu) and np.array_equal(pu, bu) and all(
+        np.array_equal(pf[name], bf[name]) for name in ("p", "k", "n", "c"))
+    return {"units": n, "seed": ppsd_units.MIX_SEED, "primes": st.primes, "window_steps": st.window_steps,
+            "factors": len(out), "batched_s": tb, "plain_s": tp, "batched_primes_per_s": st.primes / tb,
+            "plain_primes_per_s": st.primes / tp, "batched_over_plain": tp / tb, "identical_per_unit": bool(identical)}
 
 
 def main():
@@ -109,8 +150,7 @@ def main():
     value = st.primes / dt
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
     t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
-    ops = MUL32_PER_STEP * R_MUL + ALU32_PER_STEP
-    achieved = steps_per_launch * ops / t_launch / 1e12
+    achieved = steps_per_launch * OPS_PER_STEP / t_launch / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -147,6 +187,8 @@ def main():
             "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
         },
     }
+    if a.mix_units:
+        line["config"]["batched"] = mixed_units(a.mix_units)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(line), flush=True)

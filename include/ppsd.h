@@ -232,8 +232,9 @@ int ppsd_serve_close(ppsd_server *s);
 
 /* Arithmetic check (no counterpart in the reference): out[i] = 2^-e mod P[i]
  * after nwin further windows of s, i.e. 2^-(e + nwin*s) mod P[i], computed by
- * the same device code as the walk.  Requires P[i] odd in (2^32, 2^62),
- * e >= 64, 1 <= s <= 32. */
+ * the same device code as the walk, returned as the walk holds it (the
+ * walk keeps every state canonical, in [0, P[i])).  Requires P[i] odd in
+ * (2^32, 2^62), e >= 64, 1 <= s <= 32. */
 int ppsd_debug_state(int device, const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s,
                      uint64_t *out);
 

@@ -341,7 +341,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.kept_count = c->d_counters + 1;
   f.kept_cap = c->hit_cap;
   HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  HIP_TRY(launch_filter(f, 4096, c->stream));
+  HIP_TRY(launch_filter(f, 1024, c->stream));
   HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   c->st.walk_launches++;
   return 0;

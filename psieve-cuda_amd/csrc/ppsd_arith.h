@@ -69,16 +69,19 @@ __device__ __forceinline__ uint64_t inv2pow(uint64_t p, uint64_t pinv, uint32_t 
 // ceil(m*p_lo / 2^32) is the high word of m*p_lo + (2^32 - 1).  Folding the
 // 2^(32-s) into the inverse, m = K_lo * ps_sh mod 2^32 with
 // ps_sh = (-p^-1 mod 2^32) << (32-s).
-// Every term is non-negative, so there is no borrow to repair, and the state
-// stays in [1, 2p) (K < 2p gives K' < p + p/2^s): no per-step reduction.  A
-// state >= p is caught by the walk's pre-test and reduced only where a
-// candidate is examined.
+// Every term is non-negative, so there is no borrow to repair, and a
+// canonical state stays canonical: K <= p-1 and j <= 2^s - 1 give
+// K + j*p <= 2^s*p - 1, so K' <= p - 1.  No per-step reduction, and the
+// walk's pre-test never sees an unreduced state.
 template <bool S32>
 __device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s) {
   const uint32_t m = lo32(K) * ps_sh;
-  const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
+  // m*p_hi + (K >> s) first: the allocator then keeps the zero-extended
+  // operands in pairs whose high word stays zero (two moves per step, not 3)
   const uint64_t ks = S32 ? (uint64_t)hi32(K) : (K >> s);
-  return (uint64_t)m * hi32(p) + (ks + hc);
+  const uint64_t u = (uint64_t)m * hi32(p) + ks;
+  const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
+  return u + hc;
 }
 
 // The multiplier of step() for prime p and window s.
@@ -96,9 +99,8 @@ __device__ __forceinline__ uint32_t ffbl(uint32_t x) {
   return r;
 }
 
-// Necessary condition for a hit in the window whose reduced state is K: the
-// odd part of K must not exceed kmax.  (A state not yet reduced, K >= p, is
-// caught separately by the caller.)
+// Necessary condition for a hit in the window whose (canonical) state is K:
+// the odd part of K must not exceed kmax.
 __device__ __forceinline__ bool pretest(uint64_t K, uint64_t kmax) {
   const uint32_t i = ffbl(lo32(K));
   return (K >> (i & 63u)) <= kmax;

@@ -83,22 +83,22 @@ __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
   }
 }
 
-// test_factor on every raw hit of the round, one wave per hit, with the
-// rule of the hit's piece.
+// test_factor on every raw hit of the round, with the rule of the hit's piece.
+struct PieceRule {
+  const PieceDesc *pieces;
+  __device__ void operator()(const BHit &x, uint32_t &ks, uint32_t &ko, uint32_t &n0, uint32_t &n1) const {
+    const PieceDesc &P = pieces[x.piece];
+    ks = P.kstep;
+    ko = P.koffset;
+    n0 = P.nmin;
+    n1 = P.nmax;
+  }
+};
+
 __global__ __launch_bounds__(256) void batch_filter_kernel(BatchArgs a) {
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = wave; h < nraw; h += nwaves) {
-    const BHit x = a.hits[h];
-    const PieceDesc P = a.pieces[x.piece];
-    const bool keep = keep_factor(x.k, x.n, x.c, P.kstep, P.koffset, P.nmin, P.nmax, a.small_primes, a.nsmall);
-    if (keep && __lane_id() == 0) {
-      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
-      if (idx < a.kept_cap) a.kept[idx] = x;
-    }
-  }
+  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.nsmall, PieceRule{a.pieces});
 }
 
 // Copies the running prime count and sum after a piece was sieved.

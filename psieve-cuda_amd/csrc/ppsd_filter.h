@@ -19,13 +19,16 @@ __device__ __forceinline__ uint32_t mulmod_q(uint32_t x, uint32_t y, uint32_t q,
   return (uint32_t)r;
 }
 
-// q | k*2^n + c ?
+// q | k*2^n + c ?  2^n mod q via 2^(n mod (q-1)) (Fermat; q an odd prime),
+// so at most 15 squarings.
 __device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, uint32_t q) {
   const float rq = __frcp_rn((float)q);
+  const uint32_t e = n % (q - 1);
   uint32_t t = 1;
-  for (int b = 31 - __clz((int)n); b >= 0; --b) {
+#pragma unroll
+  for (int b = 14; b >= 0; --b) {
     t = mulmod_q(t, t, q, rq);
-    if ((n >> b) & 1u) {
+    if ((e >> b) & 1u) {
       t <<= 1;
       t -= (t >= q) ? q : 0;
     }
@@ -63,25 +66,65 @@ __device__ __forceinline__ bool pre_rules(uint64_t k, uint32_t n, int32_t c, uin
   return keep;
 }
 
-// Whole rule for one hit, evaluated by one wave (lanes split the small primes);
-// returns the same value in every lane.
-__device__ __forceinline__ bool keep_factor(uint64_t k, uint32_t n, int32_t c, uint32_t kstep, uint32_t koffset,
-                                            uint32_t nmin, uint32_t nmax, const uint32_t *small_primes,
-                                            uint32_t nsmall) {
+// The rule's cheap part for one hit, per lane: k in the residue class, n in
+// range, the residue pre-tests.
+__device__ __forceinline__ bool rule_cheap(uint64_t k, uint32_t n, int32_t c, uint32_t kstep, uint32_t koffset,
+                                           uint32_t nmin, uint32_t nmax) {
   const uint64_t b = k / kstep;
   bool keep = k == (uint64_t)kstep * b + koffset && n >= nmin && n < nmax;
   uint64_t mod31 = 1;
   if (keep) keep = pre_rules(k, n, c, mod31) && (uint32_t)(mod31 % 31) != 0;
-  if (keep) {
-    // 64 primes per pass, smallest first; most composites fall in the first
-    // passes, so the wave stops at the first pass that finds a divisor
-    for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
-      const uint32_t j = j0 + __lane_id();
-      const bool d = j < nsmall && divides(k, n, c, small_primes[j]);
-      if (__ballot(d) != 0) return false;
+  return keep;
+}
+
+// try_all_factors for one (k, n, c) held by every lane of the wave: the
+// lanes split the small primes, 64 per pass, smallest first; most composites
+// fall in the first passes, so the wave stops at the first pass that finds a
+// divisor.  Returns the same value in every lane.
+__device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t c, const uint32_t *small_primes,
+                                                uint32_t nsmall) {
+  for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
+    const uint32_t j = j0 + __lane_id();
+    const bool d = j < nsmall && divides(k, n, c, small_primes[j]);
+    if (__ballot(d) != 0) return false;
+  }
+  return true;
+}
+
+// Filters raw hits [0, nraw) into a kept ring.  Each wave takes 64 hits at a
+// time, one per lane, applies the cheap part of the rule lane-parallel, then
+// runs the trial division wave-wide for each survivor in turn; a kept hit is
+// written by the lane that holds it.  rule(x, kstep, koffset, nmin, nmax)
+// gives the hit's rule.
+template <class HitT, class Rule>
+__device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, HitT *kept,
+                                            unsigned long long *kept_count, uint64_t kept_cap,
+                                            const uint32_t *small_primes, uint32_t nsmall, const Rule &rule) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wave * 64; base < nraw; base += nwaves * 64) {
+    const uint64_t h = base + lane;
+    const bool valid = h < nraw;
+    HitT x{};
+    if (valid) x = hits[h];
+    uint32_t kstep = 2, koffset = 1, nmin = 0, nmax = 0;
+    if (valid) rule(x, kstep, koffset, nmin, nmax);
+    const bool cand = valid && rule_cheap(x.k, x.n, x.c, kstep, koffset, nmin, nmax);
+    uint64_t mask = __ballot(cand);
+    while (mask) {
+      const int j = __ffsll((unsigned long long)mask) - 1;
+      mask &= mask - 1;
+      const uint64_t k = __shfl(x.k, j);
+      const uint32_t n = __shfl(x.n, j);
+      const int32_t c = __shfl(x.c, j);
+      const bool keep = no_small_factor(k, n, c, small_primes, nsmall);
+      if (keep && (int)lane == j) {
+        const unsigned long long idx = atomicAdd(kept_count, 1ull);
+        if (idx < kept_cap) kept[idx] = x;
+      }
     }
   }
-  return keep;
 }
 
 }  // namespace ppsd

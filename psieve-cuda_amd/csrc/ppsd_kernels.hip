@@ -75,21 +75,23 @@ hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid,
 }
 
 // ---------------------------------------------------------------------------
-// filter: test_factor (pps/app.c:159-186) on every raw hit, one wave per hit
+// filter: test_factor (pps/app.c:159-186) on every raw hit
 // ---------------------------------------------------------------------------
+struct PlainRule {
+  uint32_t kstep, koffset, nmin, nmax;
+  __device__ void operator()(const Hit &, uint32_t &ks, uint32_t &ko, uint32_t &n0, uint32_t &n1) const {
+    ks = kstep;
+    ko = koffset;
+    n0 = nmin;
+    n1 = nmax;
+  }
+};
+
 __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = wave; h < nraw; h += nwaves) {
-    const Hit x = a.hits[h];
-    const bool keep = keep_factor(x.k, x.n, x.c, a.kstep, a.koffset, a.nmin, a.nmax, a.small_primes, a.nsmall);
-    if (keep && __lane_id() == 0) {
-      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
-      if (idx < a.kept_cap) a.kept[idx] = x;
-    }
-  }
+  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.nsmall,
+              PlainRule{a.kstep, a.koffset, a.nmin, a.nmax});
 }
 
 hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st) {
@@ -294,7 +296,8 @@ hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st) {
 }
 
 // 2^-e mod p followed by nwin windows of s: the arithmetic alone, for checks
-// against closed forms at any p.
+// against closed forms at any p.  The state is returned as the walk holds it
+// (no final reduction), so the checks also see that it stays canonical.
 __global__ __launch_bounds__(256) void state_kernel(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin,
                                                     uint32_t s, uint64_t *out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,8 +306,11 @@ __global__ __launch_bounds__(256) void state_kernel(const uint64_t *P, uint32_t 
   const uint64_t pinv = inv64(p);
   const uint32_t pinv_sh = step_multiplier(pinv, s);
   uint64_t K = inv2pow(p, pinv, e);
-  for (uint32_t w = 0; w < nwin; ++w) K = step<false>(K, p, pinv_sh, s);
-  out[i] = K >= p ? K - p : K;
+  if (s == 32)
+    for (uint32_t w = 0; w < nwin; ++w) K = step<true>(K, p, pinv_sh, s);
+  else
+    for (uint32_t w = 0; w < nwin; ++w) K = step<false>(K, p, pinv_sh, s);
+  out[i] = K;
 }
 
 hipError_t launch_state(const uint64_t *P, uint32_t n, uint32_t e, uint32_t nwin, uint32_t s, uint64_t *out,

@@ -58,13 +58,12 @@ __device__ __forceinline__ bool exact(uint64_t K, uint32_t n, const WindowRule &
   return k <= a.kmax && k >= a.kmin && nn < a.nmax;
 }
 
-// Pre-test of one window: a state not yet reduced (K >= p), or an odd part
-// <= kmax for the walked sign (both signs: also for p - K).
+// Pre-test of one window on the canonical state K: an odd part <= kmax for
+// the walked sign (both signs: also for p - K).
 template <int SIGN>
 __device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
-  const bool unreduced = K >= p;
-  if (SIGN == 0) return (int)unreduced | (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
-  return (int)unreduced | (int)pretest(K, kmax);
+  if (SIGN == 0) return (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
+  return pretest(K, kmax);
 }
 
 // Per-lane constants of the R primes a lane walks, and a snapshot of their
@@ -88,7 +87,7 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
   for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint64_t Kc = S.K[r] >= L.p[r] ? S.K[r] - L.p[r] : S.K[r];
+      const uint64_t Kc = S.K[r];
       uint64_t k = 0;
       uint32_t nn = 0;
       if (SIGN == 0) {

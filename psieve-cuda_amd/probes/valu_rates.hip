@@ -63,6 +63,33 @@ __global__ void k_subco(uint32_t *out, int iters, uint64_t *clk) {
   out[blockIdx.x*blockDim.x+threadIdx.x] = a0^a1^a2^a3^a4^a5^a6^a7;
   if (threadIdx.x==0 && blockIdx.x==0) { clk[0] = t1 - t0; clk[1] = r1 - r0; } }
 
+// more 32-bit forms: compares into VCC or an SGPR pair, selects, logic,
+// funnel shift, three-operand adds
+#define CHK(INS, A) asm volatile(INS : "+v"(A) : "v"(b) : "vcc", "s40", "s41");
+#define BODYK(INS) CHK(INS,a0) CHK(INS,a1) CHK(INS,a2) CHK(INS,a3) CHK(INS,a4) CHK(INS,a5) CHK(INS,a6) CHK(INS,a7)
+#define K32C(NAME, INS) \
+__global__ void NAME(uint32_t *out, int iters, uint64_t *clk) { \
+  uint32_t a0=threadIdx.x,a1=a0+1,a2=a0+2,a3=a0+3,a4=a0+4,a5=a0+5,a6=a0+6,a7=a0+7, b=blockIdx.x|1; \
+  uint64_t t0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime(); \
+  for (int i = 0; i < iters; ++i) { BODYK(INS) BODYK(INS) BODYK(INS) BODYK(INS) } \
+  uint64_t t1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime(); \
+  out[blockIdx.x*blockDim.x+threadIdx.x] = a0^a1^a2^a3^a4^a5^a6^a7; \
+  if (threadIdx.x==0 && blockIdx.x==0) { clk[0] = t1 - t0; clk[1] = r1 - r0; } }
+K32C(k_cmp32vcc, "v_cmp_le_u32 vcc, %0, %1")
+K32C(k_cmp32s, "v_cmp_le_u32_e64 s[40:41], %0, %1")
+K32C(k_cndmask, "v_cndmask_b32 %0, %0, %1, vcc")
+K32C(k_and, "v_and_b32 %0, %0, %1")
+K32C(k_sub, "v_sub_u32 %0, %0, %1")
+K32C(k_xad, "v_xad_u32 %0, %0, %1, %0")
+K32C(k_add3, "v_add3_u32 %0, %0, %1, %0")
+K32C(k_alignbit, "v_alignbit_b32 %0, %0, %1, 5")
+K32C(k_lshr32, "v_lshrrev_b32 %0, 3, %0")
+K32C(k_addco, "v_add_co_u32 %0, vcc, %0, %1")
+K32C(k_addc, "v_addc_co_u32 %0, vcc, %0, %1, vcc")
+K32C(k_min, "v_min_u32 %0, %0, %1")
+K32C(k_lshlor, "v_lshl_or_b32 %0, %0, 1, %1")
+K32C(k_bfe, "v_bfe_u32 %0, %0, %1, 7")
+
 typedef void (*kfn)(uint32_t*, int, uint64_t*);
 int main() {
   hipDeviceProp_t prop; (void)hipGetDeviceProperties(&prop, 0);
@@ -74,7 +101,11 @@ int main() {
     {"v_add_u32",k_add,32},{"v_xor_b32",k_xor,32},{"v_mov_b32",k_mov,32},{"v_mul_lo_u32",k_mullo,32},
     {"v_mul_hi_u32",k_mulhi,32},{"v_ffbl_b32",k_ffbl,32},{"add+mul_lo",k_addmul,64},
     {"v_lshrrev_b64",k_lshr64,32},{"v_mad_u64_u32",k_mad64,32},{"v_lshl_add_u64",k_lshladd64,32},
-    {"v_cmp_le_u64",k_cmp64,32},{"v_sub_co_u32",k_subco,32}};
+    {"v_cmp_le_u64",k_cmp64,32},{"v_sub_co_u32",k_subco,32},
+    {"v_cmp_le_u32 vcc",k_cmp32vcc,32},{"v_cmp_le_u32 sgpr",k_cmp32s,32},{"v_cndmask_b32",k_cndmask,32},
+    {"v_and_b32",k_and,32},{"v_sub_u32",k_sub,32},{"v_xad_u32",k_xad,32},{"v_add3_u32",k_add3,32},
+    {"v_alignbit_b32",k_alignbit,32},{"v_lshrrev_b32",k_lshr32,32},{"v_add_co_u32",k_addco,32},
+    {"v_addc_co_u32",k_addc,32},{"v_min_u32",k_min,32},{"v_lshl_or_b32",k_lshlor,32},{"v_bfe_u32",k_bfe,32}};
   hipEvent_t e0, e1; (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
   for (int wpsimd : {4, 8}) {
     const int grid = cus * wpsimd;  // 4 waves per workgroup: wpsimd waves per SIMD

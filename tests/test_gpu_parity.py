@@ -44,6 +44,23 @@ def test_state_vectors_against_pow():
         assert got.tolist() == want, (e, nwin, s)
 
 
+def test_every_state_stays_canonical():
+    """Each of the first 160 states of a walk equals 2^-n mod p exactly (not
+    merely mod p): the walk's pre-test relies on canonical states.  Primes
+    whose low word is extreme stress the ceiling term of the step."""
+    rng = random.Random(6)
+    P = [sympy.nextprime(rng.randrange(2**32, 2**62)) for _ in range(100)]
+    for hi in (1, 2**20, 2**29, 2**30 - 1):
+        P += [sympy.prevprime(hi * 2**32 + 2**32 - 1), sympy.nextprime(hi * 2**32)]
+    P += [sympy.prevprime(2**62)]
+    P = np.array(P, dtype=np.uint64)
+    for s in (32, 31, 7, 1):
+        for nwin in range(160):
+            got = ppsd.debug_state(P, 100, nwin, s)
+            want = [pow(2, -(100 + nwin * s), int(p)) for p in P]
+            assert got.tolist() == want, (s, nwin)
+
+
 def test_many_n_raw_and_filtered():
     """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
     pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
