@@ -87,7 +87,11 @@ def mixed_units(n):
 
     units = ppsd_units.mix_units(n)
     params = [u.params for u in units]
+    import ppsd_tune
+
     ctx = ppsd.Context(params[0])
+    ppsd_tune.apply_batch(ctx)  # the autotuner's cached shape for this device, when present
+    shape = dict(zip(("r", "u", "waves", "seg_min", "seg_max"), ctx.batch_tuning()))
     ctx.batch_run(params[:4])  # first-use allocation and graph capture
     ctx.reset_stats()
     t0 = time.perf_counter()
@@ -114,7 +118,8 @@ def mixed_units(n):
         np.array_equal(pf[name], bf[name]) for name in ("p", "k", "n", "c"))
     return {"units": n, "seed": ppsd_units.MIX_SEED, "primes": st.primes, "window_steps": st.window_steps,
             "factors": len(out), "batched_s": tb, "plain_s": tp, "batched_primes_per_s": st.primes / tb,
-            "plain_primes_per_s": st.primes / tp, "batched_over_plain": tp / tb, "identical_per_unit": bool(identical)}
+            "plain_primes_per_s": st.primes / tp, "batched_over_plain": tp / tb, "identical_per_unit": bool(identical),
+            "batch_shape": shape}
 
 
 def main():
@@ -125,11 +130,16 @@ def main():
         sys.exit(2)
     import ppsd
 
+    import ppsd_tune
+
     params = ppsd.derive(ppsd.SearchParams(**WORKLOAD))
     ctx = ppsd.Context(params, batch=a.batch)
     if a.tune:
         r, u, wg = (int(x) for x in a.tune.split(","))
         ctx.set_tuning(r, u, wg)
+    else:
+        ppsd_tune.apply_plain(ctx)  # the autotuner's cached shape for this device, when present
+    shape = dict(zip(("r", "u", "wg"), ctx.tuning()))
     p = WORKLOAD["pmin"]
     for _ in range(a.warmup):
         p = ctx.submit_range(p, WORKLOAD["pmax"])
@@ -177,6 +187,7 @@ def main():
             "sieve_ms": st.sieve_ms,
             "filter_ms": st.filter_ms,
             "parallelism": "1 GPU",
+            "walk_shape": shape,
         },
         "roofline": {
             "bound": "valu",

@@ -78,6 +78,10 @@ int ppsd_derive_params(ppsd_params *p);
 /* Number of HIP devices (0 without a GPU). */
 int ppsd_device_count(void);
 
+/* Architecture name (e.g. "gfx950:sramecc+:xnack-") and compute-unit count
+ * of a device; name may be NULL. */
+int ppsd_device_info(int device, char *name, uint32_t name_len, uint32_t *cus);
+
 /* Per-worker device set-up; replaces cuda_app_init (pps/appcu.h:27,
  * pps/appcu.cu:213-404).  p normally comes from ppsd_derive_params; any p
  * with status 0, 2^32 < pmin < pmax <= 2^62, 64 <= nmin <= nmax,

@@ -107,6 +107,15 @@ int ppsd_device_count(void) {
 
 const char *ppsd_last_error(void) { return g_err.c_str(); }
 
+int ppsd_device_info(int device, char *name, uint32_t name_len, uint32_t *cus) {
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d: %s", device, hipGetErrorString(e));
+  if (name && name_len) snprintf(name, name_len, "%s", prop.gcnArchName);
+  if (cus) *cus = (uint32_t)prop.multiProcessorCount;
+  return 0;
+}
+
 // What the walk itself needs (ppsd_derive_params gives a subset of this):
 // one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
 // nmin >= 64 for the 2^-nmin set-up.
@@ -132,11 +141,23 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_bitmap);
   (void)hipFree(c->d_blockcount);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
-  if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
-  if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
-  (void)hipFree(c->b_pool);
-  (void)hipFree(c->b_pool_count);
-  (void)hipFree(c->b_pool_sum);
+  for (int i = 0; i < 2; ++i) {
+    if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
+    if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
+    (void)hipFree(c->b_pool[i]);
+    (void)hipFree(c->b_pool_count[i]);
+    (void)hipFree(c->b_pool_sum[i]);
+    (void)hipFree(c->b_off[i]);
+    (void)hipFree(c->b_sums[i]);
+    for (auto &e : c->ev2[i])
+      if (e) (void)hipEventDestroy(e);
+    if (c->h_off[i]) (void)hipHostFree(c->h_off[i]);
+    if (c->h_sums[i]) (void)hipHostFree(c->h_sums[i]);
+  }
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
   (void)hipFree(c->b_pieces);
   (void)hipFree(c->b_tasks);
   (void)hipFree(c->b_ntasks);
@@ -144,8 +165,6 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->b_hits);
   (void)hipFree(c->b_kept);
   (void)hipFree(c->b_counts);
-  (void)hipFree(c->b_off);
-  (void)hipFree(c->b_sums);
   if (c->h_pieces) (void)hipHostFree(c->h_pieces);
   if (c->h_tasks) (void)hipHostFree(c->h_tasks);
   if (c->h_ntasks) (void)hipHostFree(c->h_ntasks);
@@ -614,9 +633,18 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   c->b_piece_cap = 1u << 16;
   c->b_task_cap = 1u << 22;
   c->b_hit_cap = c->b_hit_alloc = (uint64_t)1 << 25;
-  HIP_TRY(hipMalloc(&c->b_pool, c->b_pool_cap * sizeof(uint64_t)));
-  HIP_TRY(hipMalloc(&c->b_pool_count, sizeof(uint32_t)));
-  HIP_TRY(hipMalloc(&c->b_pool_sum, sizeof(unsigned long long)));
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(hipMalloc(&c->b_pool[i], c->b_pool_cap * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&c->b_pool_count[i], sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&c->b_pool_sum[i], sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&c->b_off[i], (c->b_piece_cap + 1) * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&c->b_sums[i], (c->b_piece_cap + 1) * sizeof(unsigned long long)));
+    for (auto &e : c->ev2[i]) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipHostMalloc((void **)&c->h_off[i], (c->b_piece_cap + 1) * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_sums[i], (c->b_piece_cap + 1) * sizeof(unsigned long long),
+                          hipHostMallocDefault));
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIP_TRY(hipMalloc(&c->b_pieces, c->b_piece_cap * sizeof(PieceDesc)));
   HIP_TRY(hipMalloc(&c->b_tasks, (size_t)c->b_task_cap * sizeof(Task)));
   HIP_TRY(hipMalloc(&c->b_ntasks, sizeof(uint32_t)));
@@ -624,8 +652,6 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipMalloc(&c->b_hits, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_kept, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_counts, 2 * sizeof(unsigned long long)));
-  HIP_TRY(hipMalloc(&c->b_off, (c->b_piece_cap + 1) * sizeof(uint32_t)));
-  HIP_TRY(hipMalloc(&c->b_sums, (c->b_piece_cap + 1) * sizeof(unsigned long long)));
   HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
@@ -666,9 +692,9 @@ static uint32_t sieve_count(const ppsd_ctx *c, uint64_t hi) {
   return (uint32_t)std::min<uint64_t>(n, c->nsp);
 }
 
-static BatchArgs batch_args(ppsd_ctx *c) {
+static BatchArgs batch_args(ppsd_ctx *c, int buf) {
   BatchArgs a{};
-  a.pool = c->b_pool;
+  a.pool = c->b_pool[buf];
   a.pieces = c->b_pieces;
   a.tasks = c->b_tasks;
   a.ntasks = c->b_ntasks;
@@ -692,15 +718,15 @@ static uint32_t persist_grid(const ppsd_ctx *c) {
 }
 
 // The round's device sequence -- zero the counters and the queue head, drain
-// the queue, filter -- as one graph, captured once per kernel shape.
-static int round_graph(ppsd_ctx *c) {
+// the queue, filter -- as one graph per pool, captured once per kernel shape.
+static int round_graph(ppsd_ctx *c, int buf) {
   const int key = c->b_r * 1000000 + c->b_u * 1000 + c->b_waves;
-  if (c->b_exec && c->b_graph_key == key) return 0;
-  if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
-  if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
-  c->b_exec = nullptr;
-  c->b_graph = nullptr;
-  const BatchArgs a = batch_args(c);
+  if (c->b_exec[buf] && c->b_graph_key[buf] == key) return 0;
+  if (c->b_exec[buf]) (void)hipGraphExecDestroy(c->b_exec[buf]);
+  if (c->b_graph[buf]) (void)hipGraphDestroy(c->b_graph[buf]);
+  c->b_exec[buf] = nullptr;
+  c->b_graph[buf] = nullptr;
+  const BatchArgs a = batch_args(c, buf);
   const uint32_t grid = persist_grid(c);
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -712,12 +738,11 @@ static int round_graph(ppsd_ctx *c) {
   const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (e != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error capturing the round: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error ending capture: %s", hipGetErrorString(e2));
-  c->b_graph = g;
-  HIP_TRY(hipGraphInstantiate(&c->b_exec, g, nullptr, nullptr, 0));
-  c->b_graph_key = key;
+  c->b_graph[buf] = g;
+  HIP_TRY(hipGraphInstantiate(&c->b_exec[buf], g, nullptr, nullptr, 0));
+  c->b_graph_key[buf] = key;
   return 0;
 }
-
 
 // Tasks of one round over the sieved pieces; the segment length halves from
 // b_seg_max down to b_seg_min until the queue holds enough tasks to keep
@@ -751,13 +776,13 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
 
 // Run tasks [0, n) of h_tasks through the round graph; re-run halves when
 // the raw ring overflows.  Appends kept (or raw) hits to b_result.
-static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw) {
+static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int buf) {
   std::vector<std::pair<uint32_t, uint32_t>> todo{{0, n}};
   while (!todo.empty()) {
     auto [f, k] = todo.back();
     todo.pop_back();
     if (k == 0) continue;
-    int rc = round_graph(c);
+    int rc = round_graph(c, buf);
     if (rc) return rc;
     memcpy(c->h_tasks, tasks + f, (size_t)k * sizeof(Task));
     *c->h_ntasks = k;
@@ -765,7 +790,7 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw) {
     HIP_TRY(hipMemcpyAsync(c->b_ntasks, c->h_ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     unsigned long long cnt[2];
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    HIP_TRY(hipGraphLaunch(c->b_exec, c->stream));
+    HIP_TRY(hipGraphLaunch(c->b_exec[buf], c->stream));
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     HIP_TRY(hipMemcpyAsync(cnt, c->b_counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -794,15 +819,15 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw) {
   return 0;
 }
 
-// One round: sieve the pieces into the pool, build and drain the queue.
-// Returns 1 (no error) when the pool overflowed and the caller must split.
-int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
-                     const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit) {
+// Sieve pieces [b, e) back to back into pool `buf` on stream st, with a
+// snapshot of the running count and sum after each piece (asynchronous).
+int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e, int buf, hipStream_t st,
+                      hipEvent_t ev0, hipEvent_t ev1) {
   const uint32_t np = (uint32_t)(e - b);
-  HIP_TRY(launch_zero_u32(c->b_pool_count, 1, c->stream));
-  HIP_TRY(launch_zero_u64(c->b_pool_sum, 1, c->stream));
-  HIP_TRY(launch_snapshot(c->b_pool_count, c->b_pool_sum, c->b_off, c->b_sums, 0, c->stream));
-  HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  HIP_TRY(launch_zero_u32(c->b_pool_count[buf], 1, st));
+  HIP_TRY(launch_zero_u64(c->b_pool_sum[buf], 1, st));
+  HIP_TRY(launch_snapshot(c->b_pool_count[buf], c->b_pool_sum[buf], c->b_off[buf], c->b_sums[buf], 0, st));
+  HIP_TRY(hipEventRecord(ev0, st));
   for (uint32_t i = 0; i < np; ++i) {
     const Piece &P = pieces[b + i];
     for (uint64_t lo = P.lo | 1; lo < P.hi;) {
@@ -815,21 +840,31 @@ int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, siz
       sa.nq = sieve_count(c, lo + 2 * nbits);
       sa.nq_small = std::min(c->nsp_small, sa.nq);
       sa.bitmap = c->d_bitmap;
-      HIP_TRY(launch_sieve(sa, c->stream));
-      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->b_pool, (uint32_t)c->b_pool_cap,
-                               c->b_pool_count, c->b_pool_sum, c->stream));
+      HIP_TRY(launch_sieve(sa, st));
+      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->b_pool[buf], (uint32_t)c->b_pool_cap,
+                               c->b_pool_count[buf], c->b_pool_sum[buf], st));
       lo += 2 * nbits;
     }
-    HIP_TRY(launch_snapshot(c->b_pool_count, c->b_pool_sum, c->b_off, c->b_sums, i + 1, c->stream));
+    HIP_TRY(launch_snapshot(c->b_pool_count[buf], c->b_pool_sum[buf], c->b_off[buf], c->b_sums[buf], i + 1, st));
   }
-  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  std::vector<uint32_t> off(np + 1);
-  std::vector<unsigned long long> sums(np + 1);
-  HIP_TRY(hipMemcpyAsync(off.data(), c->b_off, (np + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(sums.data(), c->b_sums, (np + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                         c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);
+  HIP_TRY(hipMemcpyAsync(c->h_off[buf], c->b_off[buf], (np + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(c->h_sums[buf], c->b_sums[buf], (np + 1) * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipEventRecord(ev1, st));
+  return 0;
+}
+
+// Wait for the sieve of pieces [b, e) in pool `buf`, then build and drain the
+// round's task queue on c->stream.  Returns 1 (no error, nothing recorded)
+// when the pieces overflowed the pool and the caller must split them.
+int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+                     const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit, int buf,
+                     hipEvent_t ev0, hipEvent_t ev1) {
+  const uint32_t np = (uint32_t)(e - b);
+  HIP_TRY(hipEventSynchronize(ev1));
+  c->st.sieve_ms += elapsed(ev0, ev1);
+  const uint32_t *off = c->h_off[buf];
+  const unsigned long long *sums = c->h_sums[buf];
   if (off[np] > c->b_pool_cap) return 1;
   for (uint32_t i = 0; i < np; ++i) {
     const Piece &P = pieces[b + i];
@@ -875,13 +910,21 @@ int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, siz
       e += et;
       ++k;
     }
-    int rc = run_tasks(c, tasks.data() + t, (uint32_t)k, raw);
+    int rc = run_tasks(c, tasks.data() + t, (uint32_t)k, raw, buf);
     if (rc) return rc;
     t += k;
   }
   // piece index -> caller's unit
   for (size_t h = first_hit; h < c->b_result.size(); ++h) c->b_result[h].piece = piece_unit[c->b_result[h].piece];
   return 0;
+}
+
+// One round on c->stream and pool 0: sieve, then walk (1 = pool overflow).
+int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+                    const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit) {
+  const int rc = sieve_round(c, pieces, b, e, 0, c->stream, c->ev[2], c->ev[3]);
+  if (rc) return rc;
+  return walk_round(c, pieces, b, e, par, raw, piece_unit, 0, c->ev[2], c->ev[3]);
 }
 
 int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int raw) {
@@ -923,23 +966,49 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
     }
   }
   // rounds: pieces packed in order up to 90% of the pool
-  std::vector<uint32_t> piece_unit(c->b_piece_cap);
-  size_t b = 0;
-  while (b < pieces.size()) {
+  std::vector<std::pair<size_t, size_t>> rounds;
+  for (size_t b = 0; b < pieces.size();) {
     size_t e = b;
     double est = 0;
     while (e < pieces.size() && e - b < c->b_piece_cap && (e == b || est + pieces[e].est <= 0.9 * c->b_pool_cap))
       est += pieces[e++].est;
-    rc = run_round(c, pieces, b, e, par, raw != 0, piece_unit);
-    if (rc == 1) {  // pool overflow: fewer pieces this round
-      if (e - b == 1) return set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
-      rc = run_round(c, pieces, b, b + (e - b) / 2, par, raw != 0, piece_unit);
-      if (rc) return rc < 0 ? rc : set_err(PPSD_ERR_FAILURE, "prime pool overflow");
-      b += (e - b) / 2;
-      continue;
-    }
-    if (rc) return rc;
+    rounds.push_back({b, e});
     b = e;
+  }
+  // Round i+1 is sieved on stream2 into the other pool while round i walks
+  // on the context's stream; round i+1's sieve is enqueued only after round
+  // i-1's walk (which read that pool) has finished.
+  std::vector<uint32_t> piece_unit(c->b_piece_cap);
+  auto sieve = [&](size_t i) {
+    const int buf = (int)(i & 1);
+    return sieve_round(c, pieces, rounds[i].first, rounds[i].second, buf, c->stream2, c->ev2[buf][0],
+                       c->ev2[buf][1]);
+  };
+  if (!rounds.empty()) {
+    rc = sieve(0);
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < rounds.size(); ++i) {
+    const int buf = (int)(i & 1);
+    const auto [b, e] = rounds[i];
+    HIP_TRY(hipEventSynchronize(c->ev2[buf][1]));
+    if (i + 1 < rounds.size()) {
+      rc = sieve(i + 1);
+      if (rc) return rc;
+    }
+    rc = walk_round(c, pieces, b, e, par, raw != 0, piece_unit, buf, c->ev2[buf][0], c->ev2[buf][1]);
+    if (rc == 1) {  // pool overflow (rare): this round's pieces again, in two halves, on pool `buf`
+      if (e - b == 1) return set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
+      HIP_TRY(hipStreamSynchronize(c->stream2));  // the sieve buffers are shared with round i+1's sieve
+      const size_t mid = b + (e - b) / 2;
+      for (const auto &[x, y] : {std::make_pair(b, mid), std::make_pair(mid, e)}) {
+        rc = sieve_round(c, pieces, x, y, buf, c->stream, c->ev[2], c->ev[3]);
+        if (!rc) rc = walk_round(c, pieces, x, y, par, raw != 0, piece_unit, buf, c->ev[2], c->ev[3]);
+        if (rc) return rc < 0 ? rc : set_err(PPSD_ERR_FAILURE, "prime pool overflow");
+      }
+    } else if (rc) {
+      return rc;
+    }
   }
   return 0;
 }
@@ -1001,7 +1070,7 @@ int ppsd_batch_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
   if (cap < 1 || cap > c->b_hit_alloc)
     return set_err(PPSD_ERR_INVALID_PARAM, "capacity must be 1..%llu", (unsigned long long)c->b_hit_alloc);
   c->b_hit_cap = cap;
-  c->b_graph_key = -1;  // the graph holds the capacity in its kernel arguments
+  c->b_graph_key[0] = c->b_graph_key[1] = -1;  // the graphs hold the capacity in their kernel arguments
   return 0;
 }
 

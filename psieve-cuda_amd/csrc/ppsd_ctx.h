@@ -88,9 +88,15 @@ struct ppsd_ctx {
   // batched walk (allocated on first use)
   bool b_ready = false;
   uint64_t b_pool_cap = 0;
-  uint64_t *b_pool = nullptr;
-  uint32_t *b_pool_count = nullptr;
-  unsigned long long *b_pool_sum = nullptr;
+  // two prime pools: the next round is sieved (on stream2) while the
+  // current one is walked (on stream)
+  uint64_t *b_pool[2] = {};
+  uint32_t *b_pool_count[2] = {};
+  unsigned long long *b_pool_sum[2] = {};
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev2[2][2] = {};                  // per pool: sieve begin/end on stream2
+  uint32_t *h_off[2] = {};                    // per pool, pinned: copies of b_off/b_sums
+  unsigned long long *h_sums[2] = {};
   uint32_t b_piece_cap = 0, b_task_cap = 0;
   PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
   Task *b_tasks = nullptr, *h_tasks = nullptr;
@@ -98,11 +104,11 @@ struct ppsd_ctx {
   BHit *b_hits = nullptr, *b_kept = nullptr;
   uint64_t b_hit_cap = 0, b_hit_alloc = 0;
   unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
-  uint32_t *b_off = nullptr;
-  unsigned long long *b_sums = nullptr;
-  hipGraph_t b_graph = nullptr;
-  hipGraphExec_t b_exec = nullptr;
-  int b_graph_key = -1;
+  uint32_t *b_off[2] = {};              // per pool: prime offset of each piece
+  unsigned long long *b_sums[2] = {};   // per pool: running prime sum after each piece
+  hipGraph_t b_graph[2] = {};           // per pool: the round graph
+  hipGraphExec_t b_exec[2] = {};
+  int b_graph_key[2] = {-1, -1};
   int b_r = 1, b_u = 8, b_waves = 8;
   uint32_t b_seg_min = 2048, b_seg_max = 65536;
   std::vector<BHit> b_result;
@@ -130,5 +136,14 @@ int batch_alloc(ppsd_ctx *c);
 int ensure_sieving(ppsd_ctx *c, uint64_t pmax);
 int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
               const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit);
+// The two halves of a round, for callers that overlap them: enqueue the
+// sieve of pieces [b, e) into pool `buf` on stream st (ev0/ev1 recorded
+// around it, ev1 after its offsets reached the host); then wait for ev1 and
+// walk on c->stream (1 = the pool overflowed, nothing recorded).
+int sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e, int buf, hipStream_t st,
+                hipEvent_t ev0, hipEvent_t ev1);
+int walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
+               const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit, int buf,
+               hipEvent_t ev0, hipEvent_t ev1);
 
 }  // namespace ppsd

@@ -81,6 +81,7 @@ def _load():
     sig = {
         "ppsd_derive_params": (C.c_int, [_P(Params)]),
         "ppsd_device_count": (C.c_int, []),
+        "ppsd_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_uint32, _P(C.c_uint32)]),
         "ppsd_open": (C.c_int, [C.c_int, _P(Params), C.c_uint32, _P(C.c_void_p)]),
         "ppsd_batch_size": (C.c_uint32, [C.c_void_p]),
         "ppsd_submit_primes": (C.c_int, [C.c_void_p, _u64p, C.c_uint32]),
@@ -176,6 +177,14 @@ def manual_params(pmin, pmax, kmin, kmax, nmin, nmax, sign=1, nstep=32, kstep=2,
 
 def device_count() -> int:
     return lib().ppsd_device_count()
+
+
+def device_info(device: int = 0):
+    """(architecture name, compute units) of a device."""
+    buf = C.create_string_buffer(128)
+    cus = C.c_uint32()
+    _check(lib().ppsd_device_info(device, buf, len(buf), C.byref(cus)), "ppsd_device_info")
+    return buf.value.decode(), cus.value
 
 
 def _ptr(a: np.ndarray):

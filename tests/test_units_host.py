@@ -57,3 +57,26 @@ def test_percentiles():
     p = ppsd_serve.percentiles(list(range(101)))
     assert p == {"p50": 50.0, "p99": 99.0}
     assert ppsd_serve.percentiles([]) == {"p50": None, "p99": None}
+
+
+def test_tuning_cache_roundtrip(tmp_path, monkeypatch):
+    import ppsd_tune
+    monkeypatch.setattr(ppsd, "device_info", lambda device=0: ("gfx950:sramecc+:xnack-", 256))
+    monkeypatch.setattr(ppsd_tune, "CACHE_DIR", str(tmp_path))
+    key = ppsd_tune.device_key(0)
+    assert key.startswith("gfx950-256cu-") and len(key.split("-")[-1]) == 12
+    assert ppsd_tune.load() is None
+    entry = {"device": key, "plain": {"best": {"r": 1, "u": 8, "wg": 64}},
+             "batch": {"best": {"r": 2, "u": 4, "waves": 8, "seg_min": 2048, "seg_max": 65536}}}
+    path = ppsd_tune.save(entry, out_dir=str(tmp_path))
+    assert path.startswith(str(tmp_path)) and ppsd_tune.load() == entry
+
+    class Ctx:
+        def set_tuning(self, *a):
+            self.plain = a
+
+        def set_batch_tuning(self, *a):
+            self.batch = a
+    c = Ctx()
+    assert ppsd_tune.apply_plain(c) and ppsd_tune.apply_batch(c)
+    assert c.plain == (1, 8, 64) and c.batch == (2, 4, 8, 2048, 65536)
