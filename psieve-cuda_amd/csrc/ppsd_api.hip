@@ -131,6 +131,7 @@ static int free_ctx(ppsd_ctx *c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   (void)hipFree(c->d_P);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_counters);
@@ -154,10 +155,7 @@ static int free_ctx(ppsd_ctx *c) {
     if (c->h_off[i]) (void)hipHostFree(c->h_off[i]);
     if (c->h_sums[i]) (void)hipHostFree(c->h_sums[i]);
   }
-  if (c->stream2) {
-    (void)hipStreamSynchronize(c->stream2);
-    (void)hipStreamDestroy(c->stream2);
-  }
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   (void)hipFree(c->b_pieces);
   (void)hipFree(c->b_tasks);
   (void)hipFree(c->b_ntasks);
@@ -945,6 +943,7 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   HIP_TRY(hipSetDevice(c->device));
   int rc = batch_alloc(c);
   if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream2));  // nothing of an earlier, failed run still in flight
   if (nunits) {
     rc = ensure_sieving(c, pmax_all);
     if (rc) return rc;

@@ -32,6 +32,7 @@ struct Request {
   uint64_t need = 0;     // bytes reserved at admission
   uint64_t primes = 0, sum = 0, steps = 0;
   uint32_t slices = 0;
+  uint32_t inflight = 0;  // planned slices not yet walked
   int status = 0;
   std::vector<ppsd_factor> hits;
 
@@ -136,56 +137,51 @@ void plan_slice(ppsd_server *s, std::vector<Request *> &act, std::vector<Piece> 
   }
 }
 
-void serve_loop(ppsd_server *s) {
-  ppsd_ctx *c = s->ctx;
-  (void)hipSetDevice(c->device);
-  std::vector<Request *> act;
+// A planned slice: one piece per request, sieved into pool `buf`.
+struct Slice {
+  std::vector<Request *> reqs;
   std::vector<Piece> pieces;
   std::vector<const ppsd_params *> par;
-  std::vector<uint32_t> piece_unit(c->b_piece_cap);
-  for (;;) {
-    {
-      // admit whatever arrived; sleep only while nothing is admitted and
-      // nothing is waiting (a request may arrive at any point of this loop)
-      std::unique_lock<std::mutex> lk(s->mu);
-      for (;;) {
-        admit_locked(s);
-        if (s->stop) return;
-        if (!s->active.empty()) break;
-        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-      }
-      act = s->active;
-    }
-    plan_slice(s, act, pieces, par);
-    const size_t np = pieces.size();
-    const double t_slice = s->now();
-    for (size_t i = 0; i < np; ++i) {
-      Request *r = act[i];
-      if (r->t_start < 0) r->t_start = t_slice;
-    }
-    c->b_result.clear();
-    c->b_unit_primes.assign(np, 0);
-    c->b_unit_sums.assign(np, 0);
-    int rc = run_round(c, pieces, 0, np, par, s->cfg.raw != 0, piece_unit);
-    if (rc == 1) {  // more primes than the pool holds: halve the pieces' ranges next time
-      for (size_t i = 0; i < np; ++i) pieces[i].hi = pieces[i].lo + std::max<uint64_t>(2, (pieces[i].hi - pieces[i].lo) / 2);
-      c->b_result.clear();
-      c->b_unit_primes.assign(np, 0);
-      c->b_unit_sums.assign(np, 0);
-      rc = run_round(c, pieces, 0, np, par, s->cfg.raw != 0, piece_unit);
+  int buf = 0;
+  int rc = 0;
+  bool live = false;
+};
+
+// Walk a sieved slice and hand its results to its requests.
+void finish_slice(ppsd_server *s, Slice &sl, std::vector<uint32_t> &piece_unit) {
+  ppsd_ctx *c = s->ctx;
+  const size_t np = sl.pieces.size();
+  const double t0 = s->now();
+  c->b_result.clear();
+  c->b_unit_primes.assign(np, 0);
+  c->b_unit_sums.assign(np, 0);
+  int rc = sl.rc;
+  if (!rc) rc = walk_round(c, sl.pieces, 0, np, sl.par, s->cfg.raw != 0, piece_unit, sl.buf, c->ev2[sl.buf][0],
+                           c->ev2[sl.buf][1]);
+  if (rc == 1 && np > 1) {  // more primes than the pool holds (rare): the pieces in two halves
+    rc = (int)hipStreamSynchronize(c->stream2) ? set_err(PPSD_ERR_FAILURE, "stream wait failed") : 0;
+    const size_t mid = np / 2;
+    for (const auto &[x, y] : {std::make_pair((size_t)0, mid), std::make_pair(mid, np)}) {
+      if (!rc) rc = sieve_round(c, sl.pieces, x, y, sl.buf, c->stream, c->ev[2], c->ev[3]);
+      if (!rc) rc = walk_round(c, sl.pieces, x, y, sl.par, s->cfg.raw != 0, piece_unit, sl.buf, c->ev[2], c->ev[3]);
       if (rc == 1) rc = set_err(PPSD_ERR_FAILURE, "prime pool overflow in a slice");
     }
-    const double dt = s->now() - t_slice;
-    double steps = 0;
-    std::unique_lock<std::mutex> lk(s->mu);
+  } else if (rc == 1) {
+    rc = set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
+  }
+  const double dt = s->now() - t0;
+  double steps = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
     s->st.slices++;
     s->st.busy_s += dt;
     for (size_t i = 0; i < np; ++i) {
-      Request *r = act[i];
+      Request *r = sl.reqs[i];
       r->slices++;
+      r->inflight--;
       if (rc) {
         r->status = rc;
-        r->cur = r->prm.pmax;
+        r->cur = r->prm.pmax;  // plan nothing more for it
         continue;
       }
       const uint64_t np_i = c->b_unit_primes[i];
@@ -196,20 +192,68 @@ void serve_loop(ppsd_server *s) {
       steps += (double)w;
       s->st.primes += np_i;
       s->st.window_steps += w;
-      r->cur = pieces[i].hi;
     }
     if (!rc)
-      for (const BHit &h : c->b_result) act[h.piece]->hits.push_back(ppsd_factor{h.p, h.k, h.n, h.c});
+      for (const BHit &h : c->b_result) sl.reqs[h.piece]->hits.push_back(ppsd_factor{h.p, h.k, h.n, h.c});
     if (dt > 0 && steps > 1e9) s->rate = 0.7 * s->rate + 0.3 * (steps / dt);
-    for (size_t i = 0; i < np; ++i) {
-      Request *r = act[i];
-      if (r->cur >= r->prm.pmax) {
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
-        finish_locked(s, r);
+    for (Request *r : sl.reqs) {
+      if (r->cur >= r->prm.pmax && r->inflight == 0) {
+        auto it = std::find(s->active.begin(), s->active.end(), r);
+        if (it != s->active.end()) {
+          s->active.erase(it);
+          finish_locked(s, r);
+        }
       }
     }
-    lk.unlock();
-    s->cv_done.notify_all();
+  }
+  s->cv_done.notify_all();
+  sl.live = false;
+}
+
+// Two slices in flight: while one is walked on the context's stream, the
+// next one -- planned from what is admitted at that moment -- is sieved on
+// the second stream into the other pool.
+void serve_loop(ppsd_server *s) {
+  ppsd_ctx *c = s->ctx;
+  (void)hipSetDevice(c->device);
+  std::vector<Request *> act;
+  std::vector<uint32_t> piece_unit(c->b_piece_cap);
+  Slice slice[2];
+  int nxt = 0;  // the slice being planned; slice[nxt ^ 1] is the one to walk
+  for (;;) {
+    Slice &cur = slice[nxt ^ 1];
+    Slice &next = slice[nxt];
+    {
+      // admit whatever arrived; sleep only while nothing is admitted and
+      // nothing is in flight (a request may arrive at any point of this loop)
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (;;) {
+        admit_locked(s);
+        if (s->stop) return;
+        act.clear();
+        for (Request *r : s->active)
+          if (r->cur < r->prm.pmax) act.push_back(r);
+        if (!act.empty() || cur.live) break;
+        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+      }
+    }
+    if (!act.empty()) {
+      plan_slice(s, act, next.pieces, next.par);
+      const size_t np = next.pieces.size();
+      next.reqs.assign(act.begin(), act.begin() + np);
+      next.buf = nxt;
+      const double t = s->now();
+      for (size_t i = 0; i < np; ++i) {
+        Request *r = next.reqs[i];
+        if (r->t_start < 0) r->t_start = t;
+        r->cur = next.pieces[i].hi;
+        r->inflight++;
+      }
+      next.rc = sieve_round(c, next.pieces, 0, np, next.buf, c->stream2, c->ev2[next.buf][0], c->ev2[next.buf][1]);
+      next.live = true;
+    }
+    if (cur.live) finish_slice(s, cur, piece_unit);
+    nxt ^= 1;
   }
 }
 

@@ -85,3 +85,32 @@ def mix_units(n: int = MIX_UNITS, seed: int = MIX_SEED):
     the shorter stream)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     return [draw(rng) for _ in range(n)]
+
+
+def main():
+    """Search the first N units batched on device 0 and print one JSON line:
+    python psieve-cuda_amd/python/ppsd_units.py --run N"""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--run", type=int, default=256, help="units of the seeded mix to search")
+    ap.add_argument("--raw", action="store_true", help="raw divisors instead of filtered factors")
+    a = ap.parse_args()
+    units = mix_units(a.run)
+    params = [u.params for u in units]
+    with ppsd.Context(params[0]) as ctx:
+        t0 = time.perf_counter()
+        out, unit, cnt, sums = ctx.batch_run(params, raw=a.raw)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+    print(json.dumps({"units": a.run, "seed": MIX_SEED, "seconds": dt, "primes": st.primes,
+                      "window_steps": st.window_steps, "raw_hits": st.raw_hits, "factors": len(out),
+                      "walk_ms": st.walk_ms, "sieve_ms": st.sieve_ms, "walk_launches": st.walk_launches,
+                      "primes_per_s": st.primes / dt, "ref_rejected_units": sum(not u.ref_domain for u in units),
+                      "ref_s48_units": sum(u.ref_s48 for u in units)}))
+
+
+if __name__ == "__main__":
+    main()
