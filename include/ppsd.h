@@ -133,6 +133,15 @@ int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
  * capacity only trades memory for re-walks; the result never changes. */
 int ppsd_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
 
+/* Candidate list of a sieve file (-i; pps/app.c:181-184, read by
+ * pps/app.c:349-517): when set, ppsd_collect keeps the divisors whose
+ * (k, n) is a listed candidate instead of those passing the small-prime
+ * rule.  k = kstep*b + koffset is listed when bit (b - b0) % 8 of byte
+ * (b - b0) / 8 in row (n - n0) is set; rows rows of row_bytes bytes.  The
+ * library keeps a device copy; bitmap NULL clears it. */
+int ppsd_set_candidates(ppsd_ctx *ctx, const uint8_t *bitmap, uint32_t n0, uint32_t rows, uint64_t b0,
+                        uint64_t row_bytes);
+
 int ppsd_get_stats(const ppsd_ctx *ctx, ppsd_stats *st);
 int ppsd_reset_stats(ppsd_ctx *ctx);
 

@@ -140,6 +140,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_small);
   (void)hipFree(c->d_sp);
   (void)hipFree(c->d_bitmap);
+  (void)hipFree(c->d_cand);
   (void)hipFree(c->d_blockcount);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (int i = 0; i < 2; ++i) {
@@ -304,6 +305,30 @@ int ppsd_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
   return 0;
 }
 
+int ppsd_set_candidates(ppsd_ctx *c, const uint8_t *bitmap, uint32_t n0, uint32_t rows, uint64_t b0,
+                        uint64_t row_bytes) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "a submission is outstanding");
+  HIP_TRY(hipSetDevice(c->device));
+  (void)hipFree(c->d_cand);
+  c->d_cand = nullptr;
+  c->cand_rows = 0;
+  if (!bitmap) return 0;
+  if (!rows || !row_bytes) return set_err(PPSD_ERR_INVALID_PARAM, "empty candidate list");
+  const size_t bytes = (size_t)rows * row_bytes;
+  const hipError_t e = hipMalloc(&c->d_cand, bytes);
+  if (e != hipSuccess) {
+    c->d_cand = nullptr;
+    return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "candidate list of %zu bytes: %s", bytes, hipGetErrorString(e));
+  }
+  HIP_TRY(hipMemcpy(c->d_cand, bitmap, bytes, hipMemcpyHostToDevice));
+  c->cand_row_bytes = row_bytes;
+  c->cand_b0 = b0;
+  c->cand_n0 = n0;
+  c->cand_rows = rows;
+  return 0;
+}
+
 int ppsd_get_stats(const ppsd_ctx *c, ppsd_stats *st) {
   if (!c || !st) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
   *st = c->st;
@@ -357,6 +382,11 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.kept = c->d_kept;
   f.kept_count = c->d_counters + 1;
   f.kept_cap = c->hit_cap;
+  f.cand = c->d_cand;
+  f.cand_row_bytes = c->cand_row_bytes;
+  f.cand_b0 = c->cand_b0;
+  f.cand_n0 = c->cand_n0;
+  f.cand_rows = c->cand_rows;
   HIP_TRY(hipEventRecord(c->ev[4], c->stream));
   HIP_TRY(launch_filter(f, 1024, c->stream));
   HIP_TRY(hipEventRecord(c->ev[5], c->stream));

@@ -70,6 +70,9 @@ struct ppsd_ctx {
   uint32_t *d_blockcount = nullptr;
   uint64_t nbits_max = 0;
   uint64_t *h_stage = nullptr;  // pinned upload staging
+  uint8_t *d_cand = nullptr;    // candidate list of a sieve file (ppsd_set_candidates)
+  uint64_t cand_row_bytes = 0, cand_b0 = 0;
+  uint32_t cand_n0 = 0, cand_rows = 0;
 
   hipEvent_t ev[6] = {};  // walk begin/end, sieve begin/end, filter begin/end
 

@@ -41,6 +41,11 @@ struct FilterArgs {
   Hit *kept;
   unsigned long long *kept_count;
   uint64_t kept_cap;
+  // candidate list of a sieve file (null: the small-prime rule applies):
+  // bit (b - cand_b0) of row (n - cand_n0), k = kstep*b + koffset
+  const uint8_t *cand;
+  uint64_t cand_row_bytes, cand_b0;
+  uint32_t cand_n0, cand_rows;
 };
 
 struct SieveArgs {

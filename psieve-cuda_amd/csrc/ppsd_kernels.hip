@@ -94,8 +94,34 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
               PlainRule{a.kstep, a.koffset, a.nmin, a.nmax});
 }
 
+// With a sieve file's candidate list the rule is membership alone
+// (pps/app.c:181-184): one hit per thread.
+__global__ __launch_bounds__(256) void filter_cand_kernel(FilterArgs a) {
+  const unsigned long long raw = *a.hit_count;
+  const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; h < nraw; h += stride) {
+    const Hit x = a.hits[h];
+    const uint64_t b = x.k / a.kstep;
+    bool keep = x.k == (uint64_t)a.kstep * b + a.koffset && x.n >= a.nmin && x.n < a.nmax;
+    if (keep) {
+      const uint64_t row = (uint64_t)x.n - a.cand_n0;
+      const uint64_t bit = b - a.cand_b0;
+      keep = x.n >= a.cand_n0 && row < a.cand_rows && b >= a.cand_b0 && bit < a.cand_row_bytes * 8 &&
+             ((a.cand[row * a.cand_row_bytes + bit / 8] >> (bit % 8)) & 1u);
+    }
+    if (keep) {
+      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
+      if (idx < a.kept_cap) a.kept[idx] = x;
+    }
+  }
+}
+
 hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, st, a);
+  if (a.cand)
+    hipLaunchKernelGGL(filter_cand_kernel, dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

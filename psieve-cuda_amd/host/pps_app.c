@@ -17,6 +17,7 @@
 
 #include "../../include/ppsd.h"
 #include "pps_app.h"
+#include "pps_input.h"
 
 uint64_t kmin = 0, kmax = 0;
 unsigned int nmin = 0, nmax = 0;
@@ -31,6 +32,10 @@ static int device_opt = -1;
 static unsigned int user_cthread_count = 0;
 static unsigned int kstep = 2, koffset = 1;
 static ppsd_params params;
+/* candidate list of the -i file: rows n = nmin..nmax, bit (k - kmin)/kstep */
+static uint8_t *cand_bitmap = NULL;
+static uint64_t cand_row_bytes = 0;
+static uint32_t cand_rows = 0;
 static pthread_mutex_t factors_mutex = PTHREAD_MUTEX_INITIALIZER;
 
 #define MAX_WORKERS 32
@@ -106,7 +111,7 @@ int app_parse_option(int opt, char *arg, const char *source) {
 
 void app_help(void) {
   printf("-f --factors=FILE  Write factors to FILE (default `%s')\n", FACTORS_FILENAME_DEFAULT);
-  printf("-i --input=FILE    Read initial sieve from FILE (not supported by this build)\n");
+  printf("-i --input=FILE    Read initial sieve from FILE\n");
   printf("-M --modulus=M     Print only k's == s mod M (default 2)\n");
   printf("-s --modshift=S    (default 1, or M/2)\n");
   printf("-k --kmin=K0\n");
@@ -123,10 +128,29 @@ void app_help(void) {
  * ppsd_derive_params'; failures exit with the reference's code and text. */
 void app_init(void) {
   int rc;
+  int file_format = 0;
   print_factors = quiet_opt ? 0 : 1;
   if (input_filename != NULL) {
-    bmsg("Input sieve files (-i) are not supported by this build\n");
-    bexit(ERR_NOT_IMPLEMENTED);
+    /* pps/app.c:661-663, 188-347: the file's k, n ranges (and sign) fill
+     * what the command line left open; its format is always taken from its
+     * header */
+    pps_input_scan scan;
+    char err[300];
+    if (pps_input_scan_file(input_filename, &scan, err, sizeof err)) {
+      fprintf(stderr, "%s%s\n", bmprefix(), err);
+      bexit(ERR_SCANF);
+    }
+    file_format = scan.format;
+    if (kmin == 0 || kmax == 0 || nmin == 0 || nmax == 0) {
+      if (scan.format == PPS_INPUT_ABCD) printf("Scanning ABCD file...\n");
+      search_proth = scan.sign;
+      printf("Found K's from %" PRIu64 " to %" PRIu64 ".\n", scan.k0, scan.k1);
+      printf("Found N's from %u to %u.\n", scan.n0, scan.n1);
+      if (kmin < scan.k0) kmin = scan.k0;
+      if (kmax == 0 || kmax > scan.k1) kmax = scan.k1;
+      if (nmin < scan.n0) nmin = scan.n0;
+      if (nmax == 0 || nmax > scan.n1) nmax = scan.n1;
+    }
   }
   memset(&params, 0, sizeof params);
   params.pmin = pmin;
@@ -152,6 +176,27 @@ void app_init(void) {
   nmax = params.nmax;
   printf("nstart=%u, nstep=%u\n", nmin, params.nstep_ref);
   if (params.nstep != params.nstep_ref) printf("nstep changed to %u\n", params.nstep);
+  if (input_filename != NULL) {
+    /* pps/app.c:726-738: one row per n in [nmin, nmax], one bit per k */
+    char err[300];
+    unsigned int count = 0;
+    const uint64_t b0 = kmin / kstep, b1 = kmax / kstep;
+    cand_row_bytes = (b1 - b0 + 8) / 8;
+    cand_rows = nmax - nmin + 1;
+    cand_bitmap = calloc((size_t)cand_rows, cand_row_bytes);
+    if (!cand_bitmap) {
+      bmsg("Cannot allocate the candidate bitmap\n");
+      bexit(ERR_INSUFFICIENT_RESOURCE);
+    }
+    if (file_format == PPS_INPUT_ABCD) printf("Reading ABCD file.\n");
+    if (pps_input_read_file(input_filename, file_format, kmin, kmax, nmin, nmax, kstep, koffset, cand_bitmap,
+                            cand_row_bytes, &count, err, sizeof err)) {
+      fprintf(stderr, "%s%s\n", bmprefix(), err);
+      bexit(ERR_SCANF);
+    }
+    printf("Read %u terms from %s format input file `%s'\n", count,
+           file_format == PPS_INPUT_ABCD ? "ABCD" : "NewPGen", input_filename);
+  }
   if (factors_filename == NULL) factors_filename = FACTORS_FILENAME_DEFAULT;
   if ((factors_file = fopen(factors_filename, "a")) == NULL) {
     fprintf(stderr, "%sCannot open factors file `%s'\n", bmprefix(), factors_filename);
@@ -174,6 +219,15 @@ unsigned int app_thread_init(int th) {
     fprintf(stderr, "%sGPU %d not usable: %s\n", bmprefix(), dev, ppsd_last_error());
     if (rc == PPSD_ERR_INVALID_PARAM) bexit(ERR_INVALID_PARAM);
     return 0;
+  }
+  if (cand_bitmap) {
+    rc = ppsd_set_candidates(ctxs[th], cand_bitmap, nmin, cand_rows, kmin / kstep, cand_row_bytes);
+    if (rc) {
+      fprintf(stderr, "%sGPU %d: candidate list: %s\n", bmprefix(), dev, ppsd_last_error());
+      ppsd_close(ctxs[th]);
+      ctxs[th] = NULL;
+      return 0;
+    }
   }
   fcap[th] = 1024;
   fbuf[th] = malloc(fcap[th] * sizeof(ppsd_factor));

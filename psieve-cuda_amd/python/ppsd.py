@@ -91,6 +91,7 @@ def _load():
         "ppsd_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "ppsd_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
         "ppsd_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "ppsd_set_candidates": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64]),
         "ppsd_get_stats": (C.c_int, [C.c_void_p, _P(Stats)]),
         "ppsd_reset_stats": (C.c_int, [C.c_void_p]),
         "ppsd_close": (C.c_int, [C.c_void_p]),
@@ -251,6 +252,16 @@ class Context:
 
     def set_hit_capacity(self, cap: int):
         _check(lib().ppsd_set_hit_capacity(self._h, cap), "ppsd_set_hit_capacity")
+
+    def set_candidates(self, bitmap, n0: int = 0, b0: int = 0):
+        """Candidate list (a 2-D uint8 array: rows n0.., bit b-b0 for k =
+        kstep*b + koffset), or None to clear."""
+        if bitmap is None:
+            _check(lib().ppsd_set_candidates(self._h, None, 0, 0, 0, 0), "ppsd_set_candidates")
+            return
+        bm = np.ascontiguousarray(bitmap, dtype=np.uint8)
+        _check(lib().ppsd_set_candidates(self._h, bm.ctypes.data_as(C.c_void_p), n0, bm.shape[0], b0, bm.shape[1]),
+               "ppsd_set_candidates")
 
     def stats(self) -> Stats:
         st = Stats()

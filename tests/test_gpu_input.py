@@ -1,0 +1,55 @@
+"""Sieve-file input (-i) on the GPU: the factors written are exactly the
+divisors whose (k, n) the file lists (pps/app.c:181-184), for both file
+formats and both signs, and the same as the reference's own host program
+(linked on libppsd) writes for the same file."""
+import os
+import subprocess
+
+import pytest
+
+import pps_oracle as orc
+from conftest import REPO
+from sievefiles import candidates, write_abcd, write_newpgen
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(REPO, "psieve-cuda_amd", "host", "ppsieve")
+REF = os.path.join(REPO, "oracle", "_ref", "ppsieve_ref_on_ppsd")
+LO, HI = 10**12, 10**12 + 4 * 10**6
+
+
+def run(exe, args, cwd):
+    print(os.path.basename(exe), " ".join(args), flush=True)
+    r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    with open(os.path.join(cwd, "ppfactors.txt")) as f:
+        return {orc.parse_line(l.rstrip("\n")) for l in f if "|" in l}
+
+
+def expected(sign, nmax, seed):
+    pr = orc.derive(pmin=LO, pmax=HI, kmin=3, kmax=9999, nmin=100, nmax=nmax, sign=sign)
+    raw = orc.hit_set(orc.walk(pr, orc.primes(LO, HI), filter=False, threads=8))
+    terms, chosen = candidates(raw, 3, 9999, 100, nmax, seed)
+    want = {h for h in raw if (h[1], h[2]) in chosen}
+    assert len(raw) > 50 and 0 < len(want) < len(raw)
+    return terms, want
+
+
+@pytest.mark.parametrize("fmt,sign", [("abcd", 1), ("newpgen", -1)])
+def test_sieve_file(tmp_path, fmt, sign):
+    terms, want = expected(sign, 200000, seed=11 + sign)
+    path = os.path.join(tmp_path, "sieve.txt")
+    (write_abcd if fmt == "abcd" else write_newpgen)(path, terms, sign)
+    # ranges (and sign) from the file
+    d1 = tmp_path / "scan"
+    d1.mkdir()
+    assert run(EXE, [f"-p{LO}", f"-P{HI}", "-i", path, "-q"], str(d1)) == want
+    # every range on the command line (the sign then too)
+    d2 = tmp_path / "given"
+    d2.mkdir()
+    args = [f"-p{LO}", f"-P{HI}", "-i", path, "-k3", "-K9999", "-n100", "-N200000", "-q"] + (["-R"] if sign < 0 else [])
+    assert run(EXE, args, str(d2)) == want
+    if os.path.exists(REF):
+        d3 = tmp_path / "ref"
+        d3.mkdir()
+        assert run(REF, [f"-p{LO}", f"-P{HI}", "-i", path, "-q"], str(d3)) == want

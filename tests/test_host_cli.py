@@ -75,7 +75,7 @@ def test_config_file_errors(wd):
     (["-p1e12", "-k3", "-K99", "-n300", "-N200"], -178, "nmin <= nmax is required"),
     (["-p1e12", "-k3", "-K99", "-n32", "-N200"], -178, "nmin too small"),
     (["-p1e12", "-n100", "-N200"], -178, "Please specify"),
-    (["-p1e12", "-k3", "-K99", "-n100", "-N200", "-iabcd.txt"], -150, "not supported"),
+    (["-p1e12", "-k3", "-K99", "-n100", "-N200", "-iabcd.txt"], -140, "abcd.txt: cannot open"),
     (["-p4e18", "-P3e18", "-k3", "-K99"], 1, "pmax must be greater than pmin"),
 ])
 def test_parameter_errors(wd, args, code, msg):
