@@ -21,7 +21,11 @@
 
 uint64_t kmin = 0, kmax = 0;
 unsigned int nmin = 0, nmax = 0;
+#ifdef PPS_TWIN
+int search_proth = 0; /* both signs, always */
+#else
 int search_proth = 1; /* +1 Proth, -1 Riesel (-R), 0 both (--both) */
+#endif
 
 static const char *input_filename = NULL;
 static const char *factors_filename = NULL;
@@ -30,7 +34,7 @@ static unsigned int factor_count = 0;
 static int print_factors = 1;
 static int device_opt = -1;
 static unsigned int user_cthread_count = 0;
-static unsigned int kstep = 2, koffset = 1;
+static unsigned int kstep = KSTEP_DEFAULT, koffset = KOFFSET_DEFAULT;
 static ppsd_params params;
 /* candidate list of the -i file: rows n = nmin..nmax, bit (k - kmin)/kstep */
 static uint8_t *cand_bitmap = NULL;
@@ -102,8 +106,10 @@ int app_parse_option(int opt, char *arg, const char *source) {
       if (koffset >= kstep) koffset = kstep / 2;
       break;
     case 's': status = parse_uint(&koffset, arg, 1, (1U << 31) - 1); break;
+#ifndef PPS_TWIN
     case 'R': search_proth = -1; break;
     case 257: search_proth = 0; break;
+#endif
     case 'b': break; /* accepted and ignored, as pps/app.c:557-561 */
   }
   return status;
@@ -112,15 +118,17 @@ int app_parse_option(int opt, char *arg, const char *source) {
 void app_help(void) {
   printf("-f --factors=FILE  Write factors to FILE (default `%s')\n", FACTORS_FILENAME_DEFAULT);
   printf("-i --input=FILE    Read initial sieve from FILE\n");
-  printf("-M --modulus=M     Print only k's == s mod M (default 2)\n");
-  printf("-s --modshift=S    (default 1, or M/2)\n");
+  printf("-M --modulus=M     Print only k's == s mod M (default %u)\n", KSTEP_DEFAULT);
+  printf("-s --modshift=S    (default %u, or M/2)\n", KOFFSET_DEFAULT);
   printf("-k --kmin=K0\n");
   printf("-K --kmax=K1       Sieve for primes k*2^n+/-1 with K0 <= k <= K1\n");
   printf("-m --mthreads=M    Primes per device batch = M per compute unit\n");
   printf("-n --nmin=N0\n");
   printf("-N --nmax=N1       Sieve for primes k*2^n+/-1 with N0 <= n < N1\n");
+#ifndef PPS_TWIN
   printf("-R --riesel        Sieve for primes k*2^n-1 instead of +1.\n");
   printf("   --both          Sieve k*2^n+1 and k*2^n-1 in one pass.\n");
+#endif
   printf("-d --device=N      Use GPU N instead of 0-threads\n");
 }
 
@@ -136,14 +144,16 @@ void app_init(void) {
      * header */
     pps_input_scan scan;
     char err[300];
-    if (pps_input_scan_file(input_filename, &scan, err, sizeof err)) {
+    if (pps_input_scan_file(input_filename, PPS_INPUT_TWIN, &scan, err, sizeof err)) {
       fprintf(stderr, "%s%s\n", bmprefix(), err);
       bexit(ERR_SCANF);
     }
     file_format = scan.format;
     if (kmin == 0 || kmax == 0 || nmin == 0 || nmax == 0) {
       if (scan.format == PPS_INPUT_ABCD) printf("Scanning ABCD file...\n");
+#ifndef PPS_TWIN
       search_proth = scan.sign;
+#endif
       printf("Found K's from %" PRIu64 " to %" PRIu64 ".\n", scan.k0, scan.k1);
       printf("Found N's from %u to %u.\n", scan.n0, scan.n1);
       if (kmin < scan.k0) kmin = scan.k0;
@@ -175,7 +185,16 @@ void app_init(void) {
   nmin = params.nmin;
   nmax = params.nmax;
   printf("nstart=%u, nstep=%u\n", nmin, params.nstep_ref);
+#ifdef PPS_TWIN
+  {
+    /* the twin sieve's windows overlap by one (pps/appcu.cu:177-180); the
+     * factors do not depend on the window, so the walk keeps its own */
+    const unsigned int tw = params.nstep_ref - 1;
+    if (tw >= 32 && tw < 48 && tw != 32) printf("nstep changed to 32\n");
+  }
+#else
   if (params.nstep != params.nstep_ref) printf("nstep changed to %u\n", params.nstep);
+#endif
   if (input_filename != NULL) {
     /* pps/app.c:726-738: one row per n in [nmin, nmax], one bit per k */
     char err[300];
@@ -189,8 +208,8 @@ void app_init(void) {
       bexit(ERR_INSUFFICIENT_RESOURCE);
     }
     if (file_format == PPS_INPUT_ABCD) printf("Reading ABCD file.\n");
-    if (pps_input_read_file(input_filename, file_format, kmin, kmax, nmin, nmax, kstep, koffset, cand_bitmap,
-                            cand_row_bytes, &count, err, sizeof err)) {
+    if (pps_input_read_file(input_filename, file_format, PPS_INPUT_TWIN, kmin, kmax, nmin, nmax, kstep, koffset,
+                            cand_bitmap, cand_row_bytes, &count, err, sizeof err)) {
       fprintf(stderr, "%s%s\n", bmprefix(), err);
       bexit(ERR_SCANF);
     }

@@ -10,13 +10,25 @@
 #include <stdint.h>
 #include <stdio.h>
 
-#define APP_NAME "ppsieve"
+/* Built twice, as the reference builds ppsieve and tpsieve from one source
+ * (pps/Makefile:155-162): with PPS_TWIN the program searches k*2^n+1 and
+ * k*2^n-1 together for k = 6a+3 (the twin-prime sieve, pps/app.c:53-59). */
+#ifdef PPS_TWIN
+#define APP_PREFIX "tp"
+#define KSTEP_DEFAULT 6
+#define KOFFSET_DEFAULT 3
+#else
+#define APP_PREFIX "pp"
+#define KSTEP_DEFAULT 2
+#define KOFFSET_DEFAULT 1
+#endif
+#define APP_NAME APP_PREFIX "sieve"
 #define APP_VERSION "mi355x-0.1"
 
-#define CHECKPOINT_FILENAME "ppcheck%s.txt"   /* pps/app.h:31 */
-#define OLD_CHECKPOINT_FILENAME "ppcheckpoint.txt"
-#define CONFIG_FILENAME "ppconfig.txt"        /* pps/app.h:34 */
-#define FACTORS_FILENAME_DEFAULT "ppfactors.txt" /* pps/app.h:36 */
+#define CHECKPOINT_FILENAME APP_PREFIX "check%s.txt" /* pps/app.h:31 */
+#define OLD_CHECKPOINT_FILENAME APP_PREFIX "checkpoint.txt"
+#define CONFIG_FILENAME APP_PREFIX "config.txt"        /* pps/app.h:34 */
+#define FACTORS_FILENAME_DEFAULT APP_PREFIX "factors.txt" /* pps/app.h:36 */
 
 #define APP_SHORT_OPTS "k:K:n:N:i:f:b:d:m:M:s:R"
 #define APP_LONG_OPTS                                                                                    \

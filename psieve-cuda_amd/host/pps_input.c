@@ -5,6 +5,9 @@
  * for k*2^n-1; pps/app.c:222-224), then "<k> <n>" pairs.
  * ABCD: blocks, each a header "ABCD <k>*2^$a<+|->1 [<n>]" followed by lines
  * holding the increments of n for that k (pps/app.c:242-302, 443-505).
+ * The twin-prime build reads "<p>:T:<x>:2:3" NewPGen files and ABCD blocks
+ * "ABCD (6*$a+3)*2^<n>+1 & (6*$a+3)*2^<n>-1 [<a>]" whose lines hold
+ * increments of a (k = 6a+3) at a fixed n (pps/app.c:209-212, 260-275).
  *
  * The header formats and the meaning of each term are the reference's; the
  * reader is written line by line and skips blank lines and terms outside
@@ -44,6 +47,18 @@ static int newpgen_header(const char *line, int *sign) {
   return 1;
 }
 
+/* Twin-prime forms: k = 6a+3 (the header's a), both signs, one n per block. */
+static int abcd_twin_header(const char *line, uint64_t *a, unsigned int *n) {
+  unsigned int m;
+  return sscanf(line, "ABCD (6*$a+3)*2^%u+1 & (6*$a+3)*2^%u-1 [%" SCNu64 "]", &m, n, a) == 3 && m == *n;
+}
+
+static int newpgen_twin_header(const char *line) {
+  uint64_t p0;
+  char ch;
+  return sscanf(line, "%" SCNu64 ":T:%*c:2:%c", &p0, &ch) == 2 && ch == '3';
+}
+
 static int blank(const char *s) {
   while (*s && isspace((unsigned char)*s)) s++;
   return *s == 0;
@@ -67,6 +82,25 @@ static int walk_abcd(FILE *f, char **buf, size_t *cap, uint64_t k, unsigned int 
       if (visit(arg, k, n, *line)) return -1;
     } else if (!blank(s)) {
       break; /* the reference stops reading at the first line that is neither */
+    }
+  }
+  return 0;
+}
+
+/* Twin ABCD: per block a fixed n and increments of a (pps/app.c:260-275). */
+static int walk_abcd_twin(FILE *f, char **buf, size_t *cap, uint64_t a, unsigned int n, visit_fn visit, void *arg,
+                          unsigned int *line) {
+  if (visit(arg, 6 * a + 3, n, *line)) return -1;
+  while (getline(buf, cap, f) > 0) {
+    ++*line;
+    const char *s = *buf;
+    if (isdigit((unsigned char)s[0])) {
+      a += strtoull(s, NULL, 10);
+      if (visit(arg, 6 * a + 3, n, *line)) return -1;
+    } else if (abcd_twin_header(s, &a, &n)) {
+      if (visit(arg, 6 * a + 3, n, *line)) return -1;
+    } else if (!blank(s)) {
+      break;
     }
   }
   return 0;
@@ -104,7 +138,7 @@ static int scan_visit(void *arg, uint64_t k, unsigned int n, unsigned int line) 
   return 0;
 }
 
-int pps_input_scan_file(const char *fn, pps_input_scan *out, char *err, size_t errlen) {
+int pps_input_scan_file(const char *fn, int twin, pps_input_scan *out, char *err, size_t errlen) {
   FILE *f = fopen(fn, "r");
   char *buf = NULL;
   size_t cap = 0;
@@ -116,6 +150,17 @@ int pps_input_scan_file(const char *fn, pps_input_scan *out, char *err, size_t e
   if (!f) return fail(err, errlen, "%s: cannot open", fn);
   memset(out, 0, sizeof *out);
   if (getline(&buf, &cap, f) <= 0) {
+    rc = fail(err, errlen, "Invalid header in input file `%s'", fn);
+  } else if (twin && abcd_twin_header(buf, &k, &n)) {
+    out->format = PPS_INPUT_ABCD;
+    out->sign = 1;
+    rc = walk_abcd_twin(f, &buf, &cap, k, n, scan_visit, &st, &line);
+  } else if (twin && newpgen_twin_header(buf)) {
+    out->format = PPS_INPUT_NEWPGEN;
+    out->sign = 1;
+    rc = walk_newpgen(f, scan_visit, &st);
+    if (!rc && !st.any) rc = fail(err, errlen, "Invalid line 2 in input file `%s'", fn);
+  } else if (twin) {
     rc = fail(err, errlen, "Invalid header in input file `%s'", fn);
   } else if (abcd_header(buf, &k, &out->sign, &n)) {
     out->format = PPS_INPUT_ABCD;
@@ -160,7 +205,7 @@ static int read_visit(void *arg, uint64_t k, unsigned int n, unsigned int line) 
   return 0;
 }
 
-int pps_input_read_file(const char *fn, int format, uint64_t kmin, uint64_t kmax, unsigned int nmin,
+int pps_input_read_file(const char *fn, int format, int twin, uint64_t kmin, uint64_t kmax, unsigned int nmin,
                         unsigned int nmax, unsigned int kstep, unsigned int koffset, uint8_t *bitmap,
                         uint64_t row_bytes, unsigned int *count, char *err, size_t errlen) {
   FILE *f = fopen(fn, "r");
@@ -175,9 +220,13 @@ int pps_input_read_file(const char *fn, int format, uint64_t kmin, uint64_t kmax
   if (!f) return fail(err, errlen, "%s: cannot open", fn);
   if (getline(&buf, &cap, f) <= 0) {
     rc = fail(err, errlen, "Invalid header in input file `%s'", fn);
-  } else if (format == PPS_INPUT_ABCD && abcd_header(buf, &k, &sign, &n)) {
+  } else if (twin && format == PPS_INPUT_ABCD && abcd_twin_header(buf, &k, &n)) {
+    rc = walk_abcd_twin(f, &buf, &cap, k, n, read_visit, &st, &line);
+  } else if (twin && format == PPS_INPUT_NEWPGEN && newpgen_twin_header(buf)) {
+    rc = walk_newpgen(f, read_visit, &st);
+  } else if (!twin && format == PPS_INPUT_ABCD && abcd_header(buf, &k, &sign, &n)) {
     rc = walk_abcd(f, &buf, &cap, k, n, read_visit, &st, &line);
-  } else if (format == PPS_INPUT_NEWPGEN && newpgen_header(buf, &sign)) {
+  } else if (!twin && format == PPS_INPUT_NEWPGEN && newpgen_header(buf, &sign)) {
     rc = walk_newpgen(f, read_visit, &st);
   } else {
     rc = fail(err, errlen, "Invalid header in input file `%s'", fn);

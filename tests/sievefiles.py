@@ -38,3 +38,25 @@ def candidates(hits, kmin, kmax, nmin, nmax, seed, decoys=3000, keep=0.5):
         k = rng.randrange(kmin | 1, kmax + 1, 2)
         terms.add((k, rng.randrange(nmin, nmax)))
     return sorted(terms), chosen
+
+
+def write_abcd_twin(path, terms, comment=" // Sieved to 1000000"):
+    """The twin-prime ABCD form: per n a header with the first a (k = 6a+3),
+    then one line per further k holding the increment of a."""
+    byn = {}
+    for k, n in terms:
+        assert k % 6 == 3
+        byn.setdefault(n, set()).add((k - 3) // 6)
+    with open(path, "w") as f:
+        for n in sorted(byn):
+            a = sorted(byn[n])
+            f.write(f"ABCD (6*$a+3)*2^{n}+1 & (6*$a+3)*2^{n}-1 [{a[0]}]{comment}\n")
+            for x, y in zip(a, a[1:]):
+                f.write(f"{y - x}\n")
+
+
+def write_newpgen_twin(path, terms, p0=1000000):
+    with open(path, "w") as f:
+        f.write(f"{p0}:T:0:2:3\n")
+        for k, n in sorted(terms, key=lambda t: (t[1], t[0])):
+            f.write(f"{k} {n}\n")

@@ -71,3 +71,42 @@ def test_input_errors(wd, text, msg):
 def test_missing_input_file(wd):
     rc, _, err = run(["-p1e12", "-i", "nope.txt", "-q"], wd)
     assert rc == (-140) & 0xFF and "nope.txt: cannot open" in err
+
+
+# ---- the twin-prime build (tpsieve): k = 6a+3, both signs
+
+TP = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "psieve-cuda_amd", "host", "tpsieve")
+TWIN_TERMS = [(3, 120), (9, 120), (27, 120), (15, 130), (999, 400), (1005, 100)]
+
+
+def run_tp(args, cwd):
+    import subprocess
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([TP] + args, cwd=cwd, capture_output=True, text=True, timeout=120, env=env)
+    return r.returncode, r.stdout, r.stderr
+
+
+def test_twin_defaults(wd):
+    _, out, _ = run_tp(["-p1e12", "-k1", "-K1000", "-n100", "-N1000", "-q"], wd)
+    assert "tpsieve initialized: 3 <= k <= 999, 100 <= n < 1000" in out
+    assert os.path.exists(os.path.join(wd, "tpfactors.txt"))
+
+
+def test_twin_abcd(wd):
+    from sievefiles import write_abcd_twin
+    write_abcd_twin(os.path.join(wd, "t.txt"), TWIN_TERMS)
+    _, out, _ = run_tp(["-p1e12", "-i", "t.txt", "-q"], wd)
+    assert "Found K's from 3 to 1005." in out and "Found N's from 100 to 401." in out
+    assert "Read 6 terms from ABCD format input file `t.txt'" in out
+    assert "tpsieve initialized: 3 <= k <= 1005, 100 <= n < 401" in out
+
+
+def test_twin_newpgen(wd):
+    from sievefiles import write_newpgen_twin
+    write_newpgen_twin(os.path.join(wd, "t.txt"), TWIN_TERMS)
+    _, out, _ = run_tp(["-p1e12", "-i", "t.txt", "-q"], wd)
+    assert "Read 6 terms from NewPGen format input file `t.txt'" in out
+    # a Proth-form file is not a twin file
+    write_newpgen(os.path.join(wd, "p.txt"), TWIN_TERMS, sign=+1)
+    rc, _, err = run_tp(["-p1e12", "-i", "p.txt", "-q"], wd)
+    assert rc == (-140) & 0xFF and "Invalid header" in err
