@@ -9,6 +9,7 @@
 // the subsection holding its n, so the host's re-walk finds exactly them.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -69,6 +70,14 @@ int lg2(uint64_t v) {
   return r;
 }
 
+// The reference builds its twin-prime sieve from the same sources with
+// SEARCH_TWIN (pps/Makefile:155-162), which changes the device layer too.
+// One library serves both: the twin host sets PPSD_APPCU_TWIN=1.
+bool twin_mode() {
+  const char *e = getenv("PPSD_APPCU_TWIN");
+  return e && *e && *e != '0';
+}
+
 void device_error(const char *where) {
   fprintf(stderr, "Cuda error: %s: %s\n", where, ppsd_last_error());  // checkCUDAErr, appcu.cu:124-131
   cuda_finalize();
@@ -97,6 +106,9 @@ void cuda_init(void) {
     fprintf(stderr, "This program will work by the time pmin == %lu.\n", (unsigned long)pm);
     fail(kErrInvalidParam);
   }
+  // the twin sieve checks k*2^n+1 and k*2^n-1 with windows that overlap by
+  // one (pps/appcu.cu:177-180)
+  if (twin_mode()) ld_nstep--;
   if (ld_nstep >= 32 && ld_nstep < 48 && ((uint64_t)1 << 32) <= pmin) {
     if (ld_nstep != 32) printf("nstep changed to 32\n");
     ld_nstep = 32;
@@ -138,7 +150,7 @@ unsigned int cuda_app_init(int gpuno, int th, unsigned int cthread_count) {
   p.nmax = nmax;
   p.kstep = 2;  // raw divisors need only k odd; pps/app.c applies its own -M/-s rule
   p.koffset = 1;
-  p.sign = search_proth;
+  p.sign = twin_mode() ? 0 : search_proth;  // twin: both forms
   p.nstep_ref = ld_nstep;
   p.nstep = ld_nstep > 32 ? 32 : ld_nstep;
   if (ppsd_open(gpuno, &p, batch, &w.ctx) != 0) {

@@ -14,12 +14,14 @@ from sievefiles import candidates, write_abcd_twin, write_newpgen_twin
 pytestmark = pytest.mark.gpu
 
 TP = os.path.join(REPO, "psieve-cuda_amd", "host", "tpsieve")
+REF = os.path.join(REPO, "oracle", "_ref", "tpsieve_ref_on_ppsd")
 LO, HI = 10**12, 10**12 + 6 * 10**6
 
 
-def run(args, cwd):
-    print("tpsieve", " ".join(args), flush=True)
-    r = subprocess.run([TP] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
+def run(args, cwd, exe=TP, env=None):
+    print(os.path.basename(exe), " ".join(args), flush=True)
+    r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, **(env or {})))
     assert r.returncode == 0, r.stderr
     with open(os.path.join(cwd, "tpfactors.txt")) as f:
         return {orc.parse_line(l.rstrip("\n")) for l in f if "|" in l}
@@ -47,3 +49,14 @@ def test_twin_sieve_file(tmp_path, fmt):
     path = os.path.join(tmp_path, "t.txt")
     (write_abcd_twin if fmt == "abcd" else write_newpgen_twin)(path, terms)
     assert run([f"-p{LO}", f"-P{HI}", "-i", path, "-k3", "-K9999", "-n100", "-N200000", "-q"], str(tmp_path)) == want
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference drop-in build absent (no reference tree)")
+def test_reference_twin_host(tmp_path):
+    """The reference's own twin-prime host (main.c + pps/app.c built with
+    SEARCH_TWIN) on libppsd: its CPU re-walk (test_one_p's twin branch) and
+    filter consume the device's subsection flags; its factors are ours."""
+    want = oracle(filter=True)
+    got = run([f"-p{LO}", f"-P{HI}", "-k3", "-K9999", "-n100", "-N200000", "-q"], str(tmp_path), exe=REF,
+              env={"PPSD_APPCU_TWIN": "1"})
+    assert got == want
