@@ -5,7 +5,9 @@ Workload (single large work unit, plain kernel): Proth sign (+1), odd k in
 exact uint64 arithmetic.  One step = one batch: the primes of the next
 p-chunk are sieved on the device (2^21 primes by default), walked over the
 whole n-range, hits filtered; the step ends when the batch's factors are on
-the host.  K steps cover a prefix of the p-range.
+the host.  By default the steps cover the whole p-range (1.44e9 primes,
+about 690 steps), whose prime count and sum are checked against the
+oracle's independent sieve; --steps K times a prefix of K steps.
 
 Prints one JSON line:  value = primes/s over the timed steps (inputs
 generated on the device, nothing crosses PCIe but the factors),
@@ -42,7 +44,7 @@ TRAFFIC_BYTES_PER_PRIME = (16.0 * 2**20 + 4.6 * 2**10) / 2088811
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=0, help="batches to time (0 = the whole p-range, about 690)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0, help="primes per step (0 = library default 2^21)")
     ap.add_argument("--tune", default="", help="r,u,wg kernel shape")
@@ -145,17 +147,30 @@ def main():
         p = ctx.submit_range(p, WORKLOAD["pmax"])
         ctx.collect()
     ctx.reset_stats()
+    full = a.steps == 0
+    if full:
+        p = WORKLOAD["pmin"]  # the whole range, warm-up batches included
     p_start = p
-    factors = 0
+    factors, nsteps, count, psum = 0, 0, 0, 0
     t0 = time.perf_counter()
-    for i in range(a.steps):
+    while p < WORKLOAD["pmax"] and (full or nsteps < a.steps):
         p = ctx.submit_range(p, WORKLOAD["pmax"])
-        f, _, _ = ctx.collect()
+        f, c, s = ctx.collect()
         factors += len(f)
-        print(f"step {i}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+        count += c
+        psum = (psum + s) & (2**64 - 1)
+        nsteps += 1
+        if nsteps % 50 == 0 or not full:
+            print(f"step {nsteps}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     st = ctx.stats()
     ctx.close()
+    checksum = None
+    if full:  # the reference's count= and sum= of the whole range, from the oracle's own sieve
+        with open(os.path.join(HERE, "tests", "golden", "runs.json")) as fh:
+            g = json.load(fh)["cfg2_full"]
+        checksum = {"count": count, "sum": f"0x{psum:016x}",
+                    "matches_independent_sieve": (count, psum) == (g["count"], g["sum"])}
 
     value = st.primes / dt
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
@@ -166,9 +181,9 @@ def main():
         "value": value,
         "unit": "primes/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": nsteps,
         "warmup": a.warmup,
-        "ms_per_step": dt / a.steps * 1e3,
+        "ms_per_step": dt / max(nsteps, 1) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -176,7 +191,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain kernel, single work unit: +1, k in [1201,9999], n in [50000,6000000), "
-                        "p prime in [2^50, 2^50+5e10) (timed prefix)",
+                        "p prime in [2^50, 2^50+5e10)" + ("" if full else " (timed prefix)"),
+            "checksum": checksum,
             "p_range_timed": [p_start, p],
             "primes_timed": st.primes,
             "batch_primes": ctx.batch,

@@ -246,6 +246,83 @@ int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
   return (int64_t)n;
 }
 
+/* Count and sum (mod 2^64) of the primes in [p0, p1), p0 >= 3, for ranges
+ * too large to list: segments of 2^27 odd numbers, sieved in parallel by
+ * `threads` threads with every odd prime <= isqrt(p1-1).  The same set as
+ * orc_primes (the reference's count= and sum=, main.c:1232). */
+struct count_job {
+  uint64_t lo, p1, seg;
+  uint32_t nq;
+  const uint32_t *qs;
+  uint64_t next, count, sum;
+  pthread_mutex_t mu;
+};
+
+static void *count_worker(void *arg) {
+  struct count_job *J = arg;
+  uint8_t *buf = malloc(J->seg);
+  uint64_t count = 0, sum = 0;
+  for (;;) {
+    pthread_mutex_lock(&J->mu);
+    const uint64_t base = J->next;
+    J->next += 2 * J->seg;
+    pthread_mutex_unlock(&J->mu);
+    if (base >= J->p1) break;
+    uint64_t cnt = (J->p1 - base + 1) / 2;
+    if (cnt > J->seg) cnt = J->seg;
+    const uint64_t end = base + 2 * cnt;
+    memset(buf, 0, cnt);
+    for (uint32_t qi = 0; qi < J->nq; qi++) {
+      const uint64_t q = J->qs[qi];
+      if (q * q >= end) break;
+      uint64_t first = q * q;
+      if (first < base) {
+        const uint64_t r = base % q;
+        first = r ? base + (q - r) : base;
+        if ((first & 1) == 0) first += q;
+      }
+      for (uint64_t m = first; m < end; m += 2 * q) buf[(m - base) / 2] = 1;
+    }
+    for (uint64_t j = 0; j < cnt; j++)
+      if (!buf[j]) {
+        count++;
+        sum += base + 2 * j;
+      }
+  }
+  free(buf);
+  pthread_mutex_lock(&J->mu);
+  J->count += count;
+  J->sum += sum;
+  pthread_mutex_unlock(&J->mu);
+  return NULL;
+}
+
+int orc_count_primes(uint64_t p0, uint64_t p1, int threads, uint64_t *count, uint64_t *sum) {
+  struct count_job J;
+  pthread_t th[64];
+  if (p0 < 3 || p1 <= p0 || threads < 1 || threads > 64) return -1;
+  memset(&J, 0, sizeof J);
+  J.lo = p0 | 1;
+  J.p1 = p1;
+  J.seg = UINT64_C(1) << 27;
+  J.next = J.lo;
+  {
+    uint64_t lim = orc_isqrt(p1 - 1);
+    uint32_t *qs;
+    if (lim > (UINT64_C(1) << 32) - 1) return -1;
+    J.nq = small_sieve((uint32_t)lim, &qs);
+    J.qs = qs;
+  }
+  pthread_mutex_init(&J.mu, NULL);
+  for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, count_worker, &J);
+  for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&J.mu);
+  free((void *)J.qs);
+  *count = J.count;
+  *sum = J.sum;
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* 64-bit Montgomery arithmetic, R = 2^64                               */
 /* ------------------------------------------------------------------ */

@@ -44,6 +44,7 @@ int orc_derive(orc_params *pr);
 uint64_t orc_isqrt(uint64_t x);
 int orc_is_prime(uint64_t n);
 int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap);
+int orc_count_primes(uint64_t p0, uint64_t p1, int threads, uint64_t *count, uint64_t *sum);
 uint64_t orc_neg_inv64(uint64_t p);
 uint64_t orc_inv2pow(uint64_t p, uint32_t n);
 uint64_t orc_step(uint64_t K, uint64_t p, uint64_t ps, uint32_t s);

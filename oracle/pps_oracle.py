@@ -54,6 +54,8 @@ def lib():
         L.orc_is_prime.restype = C.c_int
         L.orc_primes.argtypes = [C.c_uint64, C.c_uint64, _u64p, C.c_uint64]
         L.orc_primes.restype = C.c_int64
+        L.orc_count_primes.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _u64p, _u64p]
+        L.orc_count_primes.restype = C.c_int
         L.orc_neg_inv64.argtypes = [C.c_uint64]
         L.orc_neg_inv64.restype = C.c_uint64
         L.orc_inv2pow.argtypes = [C.c_uint64, C.c_uint32]
@@ -86,6 +88,15 @@ def primes(p0: int, p1: int) -> np.ndarray:
     if n:
         lib().orc_primes(p0, p1, out.ctypes.data_as(_u64p), n)
     return out
+
+
+def count_primes(p0: int, p1: int, threads: int = 8):
+    """(count, sum mod 2^64) of the primes in [p0, p1), for large ranges."""
+    c, s = C.c_uint64(), C.c_uint64()
+    rc = lib().orc_count_primes(p0, p1, threads, C.byref(c), C.byref(s))
+    if rc:
+        raise ValueError("orc_count_primes: range outside its domain")
+    return c.value, s.value
 
 
 def walk(pr: OrcParams, P: np.ndarray, filter: bool = True, threads: int = 1) -> np.ndarray:

@@ -132,5 +132,31 @@ def main():
     print(json.dumps(runs, indent=1))
 
 
+FULL = {
+    # the single large unit at 2^50 and the high-p both-signs unit at 2^61
+    "cfg2_full": dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000),
+    "cfg3_full": dict(pmin=2**61, pmax=2**61 + 2 * 10**9, kmin=3, kmax=100001, nmin=100000, nmax=10000000, sign=0),
+}
+
+
+def full_counts():
+    """Prime count and sum (the reference's count=/sum=) of the full 2^50 and
+    2^61 ranges, by the oracle's threaded sieve; merged into runs.json.  Too
+    large for the oracle's walk: the GPU runs over these ranges are checked
+    by these counts and by the validity of every factor they report."""
+    path = os.path.join(HERE, "runs.json")
+    with open(path) as f:
+        runs = json.load(f)
+    for tag, kw in FULL.items():
+        cnt, psum = orc.count_primes(kw["pmin"], kw["pmax"], threads=8)
+        runs[tag] = {"params": kw, "count": cnt, "sum": psum}
+        print(tag, cnt, hex(psum), flush=True)
+    with open(path, "w") as f:
+        json.dump(runs, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--full-counts"]:
+        full_counts()
+    else:
+        main()
