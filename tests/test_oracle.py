@@ -300,3 +300,12 @@ def test_goldens_are_true_factors():
     runs = json.load(open(os.path.join(GOLDEN, "runs.json")))
     assert runs["many_n_raw"]["count"] == 1592285
     assert runs["cfg1"]["nstep"] == 27
+
+
+@pytest.mark.parametrize("p0,p1", [(3, 1000), (10**12, 10**12 + 10**7), (2**50, 2**50 + 3 * 10**7 + 7),
+                                   (2**40 + 1, 2**40 + 2**28)])
+def test_count_primes_equals_listing(p0, p1):
+    """The threaded counting sieve behind the full-range goldens agrees with
+    the listing sieve (itself pinned by sympy above) on count and sum."""
+    P = orc.primes(p0, p1)
+    assert orc.count_primes(p0, p1, threads=4) == (len(P), int(P.sum(dtype=np.uint64)))
