@@ -128,6 +128,18 @@ int ppsd_collect_raw(ppsd_ctx *ctx, ppsd_factor *out, uint64_t cap, uint64_t *n_
 int ppsd_set_tuning(ppsd_ctx *ctx, int r, int u, int wg);
 int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
 
+/* Form of the walk's per-window pre-test (no counterpart in the reference,
+ * whose kernels pick among test macros at compile time: pps/appcu.cu:779-801).
+ * 1: the odd part of the state is <= kmax (ctz, shift, compare); 2: the
+ * state is <= kmax*2^(s-1) (or, both signs, >= p minus that) -- one compare
+ * per sign, and a chunk re-walk whenever a merely small state fires it; 0
+ * (default): 2 where kmax*2^(s-1)/pmin makes such re-walks rare enough to
+ * pay, with a shorter chunk than the tuned u when that pays more, else 1.
+ * Results never depend on it.  plain_form and plain_u receive the form (1 or
+ * 2) and the chunk length this context's own walk uses. */
+int ppsd_set_pretest(ppsd_ctx *ctx, int mode);
+int ppsd_get_pretest(const ppsd_ctx *ctx, int *mode, int *plain_form, int *plain_u);
+
 /* Raw-hit ring capacity for later submissions (1 .. the allocated 2^24).
  * A batch whose raw hits overflow the ring is re-walked in halves, so the
  * capacity only trades memory for re-walks; the result never changes. */

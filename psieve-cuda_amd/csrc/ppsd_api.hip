@@ -297,6 +297,22 @@ int ppsd_get_tuning(const ppsd_ctx *c, int *r, int *u, int *wg) {
   return 0;
 }
 
+int ppsd_set_pretest(ppsd_ctx *c, int mode) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (mode < 0 || mode > 2) return set_err(PPSD_ERR_INVALID_PARAM, "pre-test mode must be 0, 1 or 2");
+  c->pre_mode = mode;
+  return 0;
+}
+
+int ppsd_get_pretest(const ppsd_ctx *c, int *mode, int *plain_form, int *plain_u) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  const WalkForm form = choose_form(c->prm, c->tune_r, c->tune_u, c->pre_mode, true);
+  if (mode) *mode = c->pre_mode;
+  if (plain_form) *plain_form = form.pre == PRE_RANGE ? 2 : 1;
+  if (plain_u) *plain_u = form.u;
+  return 0;
+}
+
 int ppsd_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
   if (cap < 1 || cap > c->hit_alloc) return set_err(PPSD_ERR_INVALID_PARAM, "capacity must be 1..%llu", (unsigned long long)c->hit_alloc);
@@ -356,6 +372,8 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   a.nmin = c->prm.nmin;
   a.nmax = c->prm.nmax;
   a.s = c->prm.nstep;
+  const WalkForm form = choose_form(c->prm, c->tune_r, c->tune_u, c->pre_mode, true);
+  a.pre = form.pre;
   a.nwin = windows(c->prm);
   a.kmin = c->prm.kmin;
   a.kmax = c->prm.kmax;
@@ -365,7 +383,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   const uint32_t per_block = (uint32_t)(c->tune_wg * c->tune_r);
   const uint32_t grid = (grid_primes + per_block - 1) / per_block;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  if (grid) HIP_TRY(launch_walk(a, c->prm.sign, c->tune_r, c->tune_u, grid, (uint32_t)c->tune_wg, c->stream));
+  if (grid) HIP_TRY(launch_walk(a, c->prm.sign, c->tune_r, form.u, grid, (uint32_t)c->tune_wg, c->stream));
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   FilterArgs f{};
   f.hits = c->d_hits;
@@ -419,6 +437,24 @@ int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
   c->last_primes = n;
   c->last_sum = sum;
   return 0;
+}
+
+ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u) {
+  if (mode == 1) return {PRE_EXACT, u};
+  if (mode == 2) return {PRE_RANGE, u};
+  // false alarms per lane-window (the smallest p of the unit fires most)
+  const double f = (p.sign == 0 ? 2.0 : 1.0) * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
+  const double base = p.sign == 0 ? 0.56 : 0.75;
+  WalkForm best{PRE_EXACT, u};
+  double best_cost = 0.97;  // the range form must win by a margin
+  for (int cu = u, halvings = 0; cu >= 4 && (free_u || cu == u); cu /= 2, ++halvings) {
+    const double cost = base + 1.95 * f * 64.0 * r * cu + 0.02 * halvings;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = {PRE_RANGE, cu};
+    }
+  }
+  return best;
 }
 
 double ppsd::hits_per_prime(const ppsd_params &p, uint64_t lo) {
@@ -910,6 +946,7 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     D.prime_off = off[i];
     D.nprimes = off[i + 1] - off[i];
     D.unit = P.unit;
+    D.pre = choose_form(u, c->b_r, c->b_u, c->pre_mode, false).pre;
     piece_unit[i] = P.unit;
     c->b_unit_primes[P.unit] += D.nprimes;
     c->b_unit_sums[P.unit] += sums[i + 1] - sums[i];

@@ -73,15 +73,24 @@ __device__ __forceinline__ uint64_t inv2pow(uint64_t p, uint64_t pinv, uint32_t 
 // canonical state stays canonical: K <= p-1 and j <= 2^s - 1 give
 // K + j*p <= 2^s*p - 1, so K' <= p - 1.  No per-step reduction, and the
 // walk's pre-test never sees an unreduced state.
+//
+// The last addition, of the 32-bit ceiling term to a 64-bit value, is a
+// v_mad_u64_u32 with multiplier 1 (add_u32_u64): the term sits in the high
+// register of the multiply that made it, and an ordinary 64-bit add would
+// first move it next to a zero register.
+__device__ __forceinline__ uint64_t add_u32_u64(uint32_t a, uint64_t b) {
+  uint64_t r, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, 1, %3" : "=v"(r), "=s"(carry) : "v"(a), "v"(b));
+  return r;
+}
+
 template <bool S32>
 __device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s) {
   const uint32_t m = lo32(K) * ps_sh;
-  // m*p_hi + (K >> s) first: the allocator then keeps the zero-extended
-  // operands in pairs whose high word stays zero (two moves per step, not 3)
   const uint64_t ks = S32 ? (uint64_t)hi32(K) : (K >> s);
   const uint64_t u = (uint64_t)m * hi32(p) + ks;
   const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
-  return u + hc;
+  return add_u32_u64(hc, u);
 }
 
 // The multiplier of step() for prime p and window s.

@@ -14,6 +14,7 @@ struct PieceDesc {
   uint32_t kstep, koffset;
   uint32_t prime_off, nprimes;  // its primes in the pool
   uint32_t unit;                // caller's unit index
+  int32_t pre;                  // pre-test form (ppsd_walk.h)
 };
 
 // A unit of the work queue: count (<= 64*R) primes from pool offset

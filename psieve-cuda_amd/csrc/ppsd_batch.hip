@@ -43,7 +43,7 @@ struct PieceSink {
   }
 };
 
-template <int SIGN, int R, int U, bool S32>
+template <int SIGN, int R, int U, bool S32, int PRE>
 __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P, const Task &T) {
   LanePrimes<R> L;
   LaneStates<R> S;
@@ -55,10 +55,10 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P,
     L.p[r] = a.pool[P.prime_off + (L.valid[r] ? first + r : T.first)];
   }
   const uint32_t n0 = P.nmin + T.w0 * P.s;
-  start_states<SIGN, R>(L, S, n0, P.s);
-  const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s};
+  const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s, P.kmax << (P.s - 1)};
+  start_states<SIGN, R>(L, S, n0, P.s, rule.bound);
   const PieceSink sink{a.hits, a.hit_count, a.hit_cap, T.piece};
-  walk_windows<SIGN, R, U, S32>(L, S, n0, T.nw, rule, sink);
+  walk_windows<SIGN, R, U, S32, PRE>(L, S, n0, T.nw, rule, sink);
 }
 
 // Every wave pulls tasks until the queue is empty; all waves reach the exit.
@@ -73,12 +73,22 @@ __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
     const Task T = a.tasks[t];
     const PieceDesc P = a.pieces[T.piece];
     const bool s32 = P.s == 32;
-    if (P.sign == 1) {
-      if (s32) run_task<1, R, U, true>(a, P, T); else run_task<1, R, U, false>(a, P, T);
-    } else if (P.sign == -1) {
-      if (s32) run_task<-1, R, U, true>(a, P, T); else run_task<-1, R, U, false>(a, P, T);
+    if (P.pre == PRE_RANGE) {
+      if (P.sign == 1) {
+        if (s32) run_task<1, R, U, true, PRE_RANGE>(a, P, T); else run_task<1, R, U, false, PRE_RANGE>(a, P, T);
+      } else if (P.sign == -1) {
+        if (s32) run_task<-1, R, U, true, PRE_RANGE>(a, P, T); else run_task<-1, R, U, false, PRE_RANGE>(a, P, T);
+      } else {
+        if (s32) run_task<0, R, U, true, PRE_RANGE>(a, P, T); else run_task<0, R, U, false, PRE_RANGE>(a, P, T);
+      }
     } else {
-      if (s32) run_task<0, R, U, true>(a, P, T); else run_task<0, R, U, false>(a, P, T);
+      if (P.sign == 1) {
+        if (s32) run_task<1, R, U, true, PRE_EXACT>(a, P, T); else run_task<1, R, U, false, PRE_EXACT>(a, P, T);
+      } else if (P.sign == -1) {
+        if (s32) run_task<-1, R, U, true, PRE_EXACT>(a, P, T); else run_task<-1, R, U, false, PRE_EXACT>(a, P, T);
+      } else {
+        if (s32) run_task<0, R, U, true, PRE_EXACT>(a, P, T); else run_task<0, R, U, false, PRE_EXACT>(a, P, T);
+      }
     }
   }
 }

@@ -14,6 +14,7 @@
 #include "../../include/ppsd.h"
 #include "ppsd_batch.h"
 #include "ppsd_kernels.h"
+#include "ppsd_walk.h"
 
 namespace ppsd {
 
@@ -51,6 +52,7 @@ struct ppsd_ctx {
   ppsd_params prm{};
   uint32_t batch = 0;
   int tune_r = 1, tune_u = 8, tune_wg = 256;
+  int pre_mode = 0;  // ppsd_set_pretest: 0 automatic, 1 exact, 2 range
 
   // device buffers
   uint64_t *d_P = nullptr;
@@ -128,6 +130,20 @@ uint32_t windows(const ppsd_params &p);
 // Expected raw hits of one prime near lo: every window holds about
 // s*(kmax-kmin)/2 candidate values out of p (both signs: twice that).
 double hits_per_prime(const ppsd_params &p, uint64_t lo);
+// How a unit is walked at kernel shape (r, u): the pre-test form (PRE_EXACT
+// or PRE_RANGE) and the chunk length.  mode 1 and 2 force a form; mode 0
+// prices both with costs measured on MI355X, relative to the exact form's
+// time per window: the range form costs 0.56 (both signs) or 0.75 (one
+// sign), plus 1.95 per expected false alarm of a wave's chunk -- states
+// <= kmax*2^(s-1) (or >= p minus that) that are no hit, 64*r*u*bound/p of
+// them (twice that for both signs) -- and 0.02 for each halving of the chunk
+// below u.  Where chunks are shared by many units (batched pieces) the chunk
+// stays u (free_u false).
+struct WalkForm {
+  int pre;
+  int u;
+};
+WalkForm choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u);
 // Milliseconds between two recorded events (0 on error).
 float elapsed(hipEvent_t a, hipEvent_t b);
 // Batched walk: buffers (allocated on first use), sieving primes for every

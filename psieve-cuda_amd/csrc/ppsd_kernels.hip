@@ -18,7 +18,7 @@ namespace ppsd {
 // ---------------------------------------------------------------------------
 // plain walk: one lane per R primes of one batch
 // ---------------------------------------------------------------------------
-template <int SIGN, int R, int U, bool S32>
+template <int SIGN, int R, int U, bool S32, int PRE>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   const uint32_t cnt = a.count_d ? *a.count_d : a.count;
   const uint32_t count = cnt < a.cap ? cnt : a.cap;
@@ -31,47 +31,50 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     L.valid[r] = first + r < count;
     L.p[r] = a.P[L.valid[r] ? first + r : count - 1];
   }
-  start_states<SIGN, R>(L, S, a.nmin, a.s);
-  const WindowRule rule{a.kmin, a.kmax, a.nmax, a.s};
+  const WindowRule rule{a.kmin, a.kmax, a.nmax, a.s, a.kmax << (a.s - 1)};
+  start_states<SIGN, R>(L, S, a.nmin, a.s, rule.bound);
   const RingSink sink{a.hits, a.hit_count, a.hit_cap};
-  walk_windows<SIGN, R, U, S32>(L, S, a.nmin, a.nwin, rule, sink);
+  walk_windows<SIGN, R, U, S32, PRE>(L, S, a.nmin, a.nwin, rule, sink);
 }
 
-template <int SIGN, int R, bool S32>
+template <int SIGN, int R, bool S32, int PRE>
 static hipError_t launch_walk_u(const WalkArgs &a, int u, uint32_t grid, uint32_t block, hipStream_t st) {
   switch (u) {
-    case 4: hipLaunchKernelGGL((walk_kernel<SIGN, R, 4, S32>), dim3(grid), dim3(block), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((walk_kernel<SIGN, R, 8, S32>), dim3(grid), dim3(block), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((walk_kernel<SIGN, R, 16, S32>), dim3(grid), dim3(block), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((walk_kernel<SIGN, R, 4, S32, PRE>), dim3(grid), dim3(block), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((walk_kernel<SIGN, R, 8, S32, PRE>), dim3(grid), dim3(block), 0, st, a); break;
+    case 16: hipLaunchKernelGGL((walk_kernel<SIGN, R, 16, S32, PRE>), dim3(grid), dim3(block), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int SIGN, bool S32>
+template <int SIGN, bool S32, int PRE>
 static hipError_t launch_walk_r(const WalkArgs &a, int r, int u, uint32_t grid, uint32_t block, hipStream_t st) {
   switch (r) {
-    case 1: return launch_walk_u<SIGN, 1, S32>(a, u, grid, block, st);
-    case 2: return launch_walk_u<SIGN, 2, S32>(a, u, grid, block, st);
+    case 1: return launch_walk_u<SIGN, 1, S32, PRE>(a, u, grid, block, st);
+    case 2: return launch_walk_u<SIGN, 2, S32, PRE>(a, u, grid, block, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <bool S32>
+template <bool S32, int PRE>
 static hipError_t launch_walk_s(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block,
                                 hipStream_t st) {
   switch (sign) {
-    case 1: return launch_walk_r<1, S32>(a, r, u, grid, block, st);
-    case -1: return launch_walk_r<-1, S32>(a, r, u, grid, block, st);
-    case 0: return launch_walk_r<0, S32>(a, r, u, grid, block, st);
+    case 1: return launch_walk_r<1, S32, PRE>(a, r, u, grid, block, st);
+    case -1: return launch_walk_r<-1, S32, PRE>(a, r, u, grid, block, st);
+    case 0: return launch_walk_r<0, S32, PRE>(a, r, u, grid, block, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block, hipStream_t st) {
   if (a.s < 1 || a.s > 32) return hipErrorInvalidValue;
-  return a.s == 32 ? launch_walk_s<true>(a, sign, r, u, grid, block, st)
-                   : launch_walk_s<false>(a, sign, r, u, grid, block, st);
+  if (a.pre == PRE_RANGE)
+    return a.s == 32 ? launch_walk_s<true, PRE_RANGE>(a, sign, r, u, grid, block, st)
+                     : launch_walk_s<false, PRE_RANGE>(a, sign, r, u, grid, block, st);
+  return a.s == 32 ? launch_walk_s<true, PRE_EXACT>(a, sign, r, u, grid, block, st)
+                   : launch_walk_s<false, PRE_EXACT>(a, sign, r, u, grid, block, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,9 +9,18 @@ namespace ppsd {
 // What a window test needs besides the state.
 struct WindowRule {
   uint64_t kmin, kmax;
-  uint32_t nmax;  // exclusive
-  uint32_t s;     // window, 1..32
+  uint32_t nmax;   // exclusive
+  uint32_t s;      // window, 1..32
+  uint64_t bound;  // kmax * 2^(s-1): no hit of a window exceeds it
 };
+
+// Pre-test forms.  PRE_EXACT: the odd part of the state (ctz, shift,
+// compare) is <= kmax.  PRE_RANGE: the state itself is <= kmax * 2^(s-1)
+// (both signs: or >= p - that), one compare per sign -- a necessary
+// condition that also fires on states that are merely small, which costs a
+// re-walk of the chunk; the host picks it when that is rare, i.e. when
+// kmax * 2^(s-1) is a tiny fraction of p.
+enum { PRE_EXACT = 0, PRE_RANGE = 1 };
 
 // Raw-hit ring of the plain walk: 24-byte records.
 struct RingSink {
@@ -59,11 +68,15 @@ __device__ __forceinline__ bool exact(uint64_t K, uint32_t n, const WindowRule &
 }
 
 // Pre-test of one window on the canonical state K: an odd part <= kmax for
-// the walked sign (both signs: also for p - K).
-template <int SIGN>
-__device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
-  if (SIGN == 0) return (int)pretest(K, kmax) | (int)pretest(p - K, kmax);
-  return pretest(K, kmax);
+// the walked sign (both signs: also for p - K); pmb = p - bound.
+template <int SIGN, int PRE>
+__device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t pmb, const WindowRule &a) {
+  if (PRE == PRE_RANGE) {
+    if (SIGN == 0) return (int)(K <= a.bound) | (int)(K >= pmb);
+    return K <= a.bound;
+  }
+  if (SIGN == 0) return (int)pretest(K, a.kmax) | (int)pretest(p - K, a.kmax);
+  return pretest(K, a.kmax);
 }
 
 // Per-lane constants of the R primes a lane walks, and a snapshot of their
@@ -71,6 +84,7 @@ __device__ __forceinline__ bool pre(uint64_t K, uint64_t p, uint64_t kmax) {
 template <int R>
 struct LanePrimes {
   uint64_t p[R];
+  uint64_t pmb[R];  // p - bound (range pre-test, both signs)
   uint32_t ps[R];
   bool valid[R];
 };
@@ -107,7 +121,7 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
 // Walk nwin windows from window start n, states S.  Steps run in chunks of U
 // and a chunk only leaves the straight-line path when one of its pre-tests
 // fired (then it is walked again with exact tests).
-template <int SIGN, int R, int U, bool S32, class Sink>
+template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
 __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                              const WindowRule &a, const Sink &sink) {
   const uint32_t s = a.s;
@@ -119,7 +133,7 @@ __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+        any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
         S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
       }
     }
@@ -129,7 +143,7 @@ __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<
   for (; w < nwin; ++w, n += s) {
     bool any = false;
 #pragma unroll
-    for (int r = 0; r < R; ++r) any |= pre<SIGN>(S.K[r], L.p[r], a.kmax);
+    for (int r = 0; r < R; ++r) any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
     if (any) rescan<SIGN, R, S32>(L, S, n, 1, a, sink);
 #pragma unroll
     for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
@@ -139,9 +153,11 @@ __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<
 // Start states: K = -c * 2^-e mod p (both signs: 2^-e, whose negation p - K
 // is the +1 form).
 template <int SIGN, int R>
-__device__ __forceinline__ void start_states(LanePrimes<R> &L, LaneStates<R> &S, uint32_t e, uint32_t s) {
+__device__ __forceinline__ void start_states(LanePrimes<R> &L, LaneStates<R> &S, uint32_t e, uint32_t s,
+                                             uint64_t bound) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
+    L.pmb[r] = L.p[r] - bound;
     const uint64_t pinv = inv64(L.p[r]);
     L.ps[r] = step_multiplier(pinv, s);
     const uint64_t x = inv2pow(L.p[r], pinv, e);

@@ -91,6 +91,8 @@ def _load():
         "ppsd_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "ppsd_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
         "ppsd_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "ppsd_set_pretest": (C.c_int, [C.c_void_p, C.c_int]),
+        "ppsd_get_pretest": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
         "ppsd_set_candidates": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64]),
         "ppsd_get_stats": (C.c_int, [C.c_void_p, _P(Stats)]),
         "ppsd_reset_stats": (C.c_int, [C.c_void_p]),
@@ -249,6 +251,16 @@ class Context:
 
     def collect_raw(self):
         return self._collect(lib().ppsd_collect_raw, "ppsd_collect_raw")
+
+    def set_pretest(self, mode: int):
+        """0 automatic, 1 exact odd-part test, 2 range test."""
+        _check(lib().ppsd_set_pretest(self._h, mode), "ppsd_set_pretest")
+
+    def pretest(self):
+        """(mode set, form the plain walk uses: 1 exact / 2 range, its chunk u)."""
+        m, f, u = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().ppsd_get_pretest(self._h, C.byref(m), C.byref(f), C.byref(u)), "ppsd_get_pretest")
+        return m.value, f.value, u.value
 
     def set_hit_capacity(self, cap: int):
         _check(lib().ppsd_set_hit_capacity(self._h, cap), "ppsd_set_hit_capacity")

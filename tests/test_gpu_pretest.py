@@ -1,0 +1,111 @@
+"""The walk's two pre-test forms (ppsd_set_pretest): the exact odd-part test
+and the range test K <= kmax*2^(s-1) (or >= p minus that).  Each is only a
+filter in front of the same exact re-scan, so every result must be the same
+under both and equal the oracle's; the range form is forced onto dense units
+where it fires in most chunks, which drives the re-scan path hard."""
+import json
+import os
+import random
+
+import pytest
+
+import pps_oracle as orc
+import ppsd
+import ppsd_units
+from conftest import GOLDEN, read_lines
+
+pytestmark = pytest.mark.gpu
+
+RUNS = json.load(open(os.path.join(GOLDEN, "runs.json")))
+EXACT, RANGE = 1, 2
+
+
+def golden_set(name):
+    return {orc.parse_line(l) for l in read_lines(name)}
+
+
+@pytest.mark.parametrize("mode", (EXACT, RANGE))
+@pytest.mark.parametrize("tag,name", [("cfg2_slice", "cfg2_slice.txt"), ("cfg3_slice", "cfg3_slice.txt"),
+                                      ("many_n", "many_n_filtered.txt")])
+def test_golden_ranges_each_form(mode, tag, name):
+    r = RUNS[tag]
+    kw = dict(r["params"])
+    with ppsd.Context(ppsd.derive(ppsd.SearchParams(**kw))) as ctx:
+        ctx.set_pretest(mode)
+        assert ctx.pretest()[:2] == (mode, mode)
+        got, cnt, psum = ctx.run_range(kw["pmin"], kw["pmax"])
+    assert ppsd.factor_set(got) == golden_set(name)
+    assert (cnt, psum) == (r["count"], r["sum"])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_dense_units_each_form(seed):
+    """Small p for its kmax (many hits per prime), every sign, s from 1 to 32,
+    each kernel shape: raw divisors and filtered factors as the oracle's."""
+    rng = random.Random(7000 + seed)
+    pmin = rng.randrange(2**32 + 1, 2**38)
+    s = (1, 3, 9, 17, 31, 32)[seed % 6]
+    kmax = max(1, min((pmin - 1) >> (s - 1), rng.randrange(2**8, 2**24)))
+    kmin = rng.randrange(1, kmax + 1)
+    nmin = rng.randrange(64, 3000)
+    nmax = nmin + rng.randrange(1, 30000)
+    sign = (1, -1, 0)[seed % 3]
+    pmax = pmin + rng.randrange(500, 15000)
+    kw = dict(pmin=pmin, pmax=pmax, kmin=kmin, kmax=kmax, nmin=nmin, nmax=nmax, sign=sign, nstep=s)
+    pr = orc.OrcParams(kstep=2, koffset=1, nstep_ref=s, **kw)
+    P = orc.primes(pmin, pmax)
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+    want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
+    with ppsd.Context(ppsd.manual_params(**kw)) as ctx:
+        ctx.set_tuning(r=1 + seed % 2, u=(4, 8, 16)[seed % 3])
+        for mode in (EXACT, RANGE):
+            ctx.set_pretest(mode)
+            raw, cnt, _ = ctx.run_range(pmin, pmax, raw=True)
+            got, _, _ = ctx.run_range(pmin, pmax)
+            assert cnt == len(P)
+            assert ppsd.factor_set(raw) == want_raw, mode
+            assert ppsd.factor_set(got) == want, mode
+
+
+def test_automatic_choice():
+    """Automatic: the range form, on a chunk of 4 windows, where
+    kmax*2^(s-1)/p is tiny (2^61, both signs, kmax 1e5: 0.05 false alarms
+    per chunk of a wave); the exact form where it is not (2^50, kmax 9999,
+    s = 32: about 10 per chunk); a forced form keeps the tuned chunk."""
+    for tag, want in (("cfg3_slice", (0, RANGE, 4)), ("cfg2_slice", (0, EXACT, 8))):
+        with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS[tag]["params"]))) as ctx:
+            ctx.set_tuning(r=1, u=8)
+            assert ctx.pretest() == want, tag
+            other = EXACT if want[1] == RANGE else RANGE
+            ctx.set_pretest(other)
+            assert ctx.pretest() == (other, other, 8)
+    with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS["cfg2_slice"]["params"]))) as ctx:
+        with pytest.raises(ppsd.PPSDError):
+            ctx.set_pretest(3)
+        with pytest.raises(ppsd.PPSDError):
+            ctx.set_pretest(-1)
+
+
+def test_batch_mix_each_form():
+    """The persistent batched walk over 24 units of the seeded mix, pieces
+    forced to each form: per-unit results equal, and equal the plain walk's."""
+    units = ppsd_units.mix_units(24)
+    params = [u.params for u in units]
+    # a small slice of each unit keeps the oracle-free comparison quick
+    for p in params:
+        p.pmax = min(p.pmax, p.pmin + 2000000)
+    per_mode = {}
+    with ppsd.Context(params[0]) as ctx:
+        for mode in (EXACT, RANGE):
+            ctx.set_pretest(mode)
+            out, unit, cnt, sums = ctx.batch_run(params, raw=True)
+            hits = {(int(u), (int(f["p"]), int(f["k"]), int(f["n"]), int(f["c"]))) for f, u in zip(out, unit)}
+            assert len(hits) == len(out)
+            per_mode[mode] = (hits, [int(x) for x in cnt], [int(x) for x in sums])
+    assert per_mode[EXACT] == per_mode[RANGE]
+    for i in (0, 5, 11, 23):
+        with ppsd.Context(params[i]) as ctx:
+            ctx.set_pretest(RANGE)
+            raw, cnt, psum = ctx.run_range(params[i].pmin, params[i].pmax, raw=True)
+        assert {t for u, t in per_mode[RANGE][0] if u == i} == ppsd.factor_set(raw)
+        assert (cnt, psum) == (per_mode[RANGE][1][i], per_mode[RANGE][2][i])
