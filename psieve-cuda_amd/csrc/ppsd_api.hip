@@ -257,12 +257,16 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   OPEN_TRY(hipMalloc(&c->d_blockcount, (c->nbits_max / 32 / kScanBlock + 2) * sizeof(uint32_t)));
   OPEN_TRY(hipHostMalloc((void **)&c->h_stage, (size_t)c->batch * sizeof(uint64_t), hipHostMallocDefault));
   {
-    std::vector<uint32_t> sm;
+    std::vector<SmallPrime> sm;
     for (uint32_t q : host_primes(32767))
-      if (q >= 11) sm.push_back(q);
+      if (q >= 11) {
+        const uint64_t two32 = (uint64_t)1 << 32;
+        sm.push_back({q, (uint32_t)(two32 / q), (uint32_t)(two32 / (q - 1)), (uint32_t)(two32 % q)});
+      }
+    while (sm.size() % 64) sm.push_back(sm.back());
     c->nsmall = (uint32_t)sm.size();
-    OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(uint32_t)));
-    OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(SmallPrime)));
+    OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(SmallPrime), hipMemcpyHostToDevice));
   }
   // Sieving primes: every odd q <= isqrt(pmax-1), or only those <= -Q when
   // given (main.c:971-973; primes up to 61 are always used, sieve.c's
@@ -406,7 +410,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.cand_n0 = c->cand_n0;
   f.cand_rows = c->cand_rows;
   HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  HIP_TRY(launch_filter(f, 1024, c->stream));
+  HIP_TRY(launch_filter(f, 2048, c->stream));
   HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   c->st.walk_launches++;
   return 0;

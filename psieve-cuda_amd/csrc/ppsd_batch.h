@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ppsd_kernels.h"
+
 namespace ppsd {
 
 // One piece (p-subrange) of a work unit, as the persistent kernel sees it.
@@ -45,7 +47,7 @@ struct BatchArgs {
   BHit *kept;
   unsigned long long *kept_count;
   uint64_t kept_cap;
-  const uint32_t *small_primes;
+  const SmallPrime *small_primes;  // padded to a multiple of 64
   uint32_t nsmall;
 };
 

@@ -62,7 +62,7 @@ struct ppsd_ctx {
   Hit *d_kept = nullptr;
   uint64_t hit_cap = 0;        // capacity in use
   uint64_t hit_alloc = 0;      // allocated entries
-  uint32_t *d_small = nullptr;  // odd primes 11 <= q < 32768
+  ppsd::SmallPrime *d_small = nullptr;  // odd primes 11 <= q < 32768, padded to a multiple of 64
   uint32_t nsmall = 0;
   uint32_t *d_sp = nullptr;     // sieving primes <= qmax
   uint32_t nsp = 0, nsp_small = 0;

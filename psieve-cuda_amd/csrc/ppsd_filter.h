@@ -5,37 +5,34 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ppsd_kernels.h"
+
 namespace ppsd {
 
-// x*y mod q for x, y < q < 2^15: the float quotient estimate is off by at most
-// one (relative error ~2^-22 on a quotient below 2^15), so two corrections
-// make it exact.
-__device__ __forceinline__ uint32_t mulmod_q(uint32_t x, uint32_t y, uint32_t q, float rq) {
-  const uint32_t prod = x * y;
-  const uint32_t e = (uint32_t)((float)prod * rq);
-  int32_t r = (int32_t)(prod - e * q);
-  r += (r < 0) ? (int32_t)q : 0;
-  r -= (r >= (int32_t)q) ? (int32_t)q : 0;
-  return (uint32_t)r;
+// x mod d for any 32-bit x, given m = floor(2^32 / d) and 1 < d < 2^31: the
+// estimate hi32(x*m) is the quotient or one less (m > 2^32/d - 1, x < 2^32),
+// so x - estimate*d lies in [0, 2d) and one unsigned min finishes it.
+__device__ __forceinline__ uint32_t mod_barrett(uint32_t x, uint32_t d, uint32_t m) {
+  const uint32_t r = x - __umulhi(x, m) * d;
+  return min(r, r - d);
 }
 
-// q | k*2^n + c ?  2^n mod q via 2^(n mod (q-1)) (Fermat; q an odd prime),
-// so at most 15 squarings.
-__device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, uint32_t q) {
-  const float rq = __frcp_rn((float)q);
-  const uint32_t e = n % (q - 1);
-  uint32_t t = 1;
+// q | k*2^n + c ?  (k < 2^62, q an odd prime below 2^15)  k mod q from its
+// two 32-bit halves; 2^n mod q as 2^e with e = n mod (q-1) (Fermat), its top
+// four bits at once -- 2^(e>>11) < 2q since e < q-1 -- then eleven
+// square-and-maybe-double steps, each product below 2q^2 < 2^31.
+__device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, const SmallPrime &sp) {
+  const uint32_t q = sp.q;
+  const uint32_t kl = mod_barrett((uint32_t)k, q, sp.mq);
+  const uint32_t kh = mod_barrett((uint32_t)(k >> 32), q, sp.mq);
+  const uint32_t km = mod_barrett(kh * sp.c32 + kl, q, sp.mq);
+  const uint32_t e = mod_barrett(n, q - 1, sp.md);
+  uint32_t t = 1u << (e >> 11);
+  t = min(t, t - q);
 #pragma unroll
-  for (int b = 14; b >= 0; --b) {
-    t = mulmod_q(t, t, q, rq);
-    if ((e >> b) & 1u) {
-      t <<= 1;
-      t -= (t >= q) ? q : 0;
-    }
-  }
-  const uint32_t km = (uint32_t)(k % q);
-  const uint32_t v = mulmod_q(km, t, q, rq);
-  return (c > 0) ? (v + 1 == q) : (v == 1);
+  for (int b = 10; b >= 0; --b) t = mod_barrett((t * t) << ((e >> b) & 1u), q, sp.mq);
+  const uint32_t v = mod_barrett(km * t, q, sp.mq);
+  return v == (c > 0 ? q - 1 : 1u);
 }
 
 // The residue pre-tests of test_factor in wrapping 64-bit arithmetic with their
@@ -80,13 +77,16 @@ __device__ __forceinline__ bool rule_cheap(uint64_t k, uint32_t n, int32_t c, ui
 // try_all_factors for one (k, n, c) held by every lane of the wave: the
 // lanes split the small primes, 64 per pass, smallest first; most composites
 // fall in the first passes, so the wave stops at the first pass that finds a
-// divisor.  Returns the same value in every lane.
-__device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t c, const uint32_t *small_primes,
+// divisor.  Each pass loads the next pass's constants before it computes.
+// nsmall is a multiple of 64.  Returns the same value in every lane.
+__device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t c, const SmallPrime *small_primes,
                                                 uint32_t nsmall) {
+  const uint32_t lane = __lane_id();
+  SmallPrime cur = small_primes[lane];
   for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
-    const uint32_t j = j0 + __lane_id();
-    const bool d = j < nsmall && divides(k, n, c, small_primes[j]);
-    if (__ballot(d) != 0) return false;
+    const SmallPrime next = small_primes[j0 + 64 < nsmall ? j0 + 64 + lane : lane];
+    if (__ballot(divides(k, n, c, cur)) != 0) return false;
+    cur = next;
   }
   return true;
 }
@@ -99,7 +99,7 @@ __device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t 
 template <class HitT, class Rule>
 __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, HitT *kept,
                                             unsigned long long *kept_count, uint64_t kept_cap,
-                                            const uint32_t *small_primes, uint32_t nsmall, const Rule &rule) {
+                                            const SmallPrime *small_primes, uint32_t nsmall, const Rule &rule) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -115,9 +115,11 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
     while (mask) {
       const int j = __ffsll((unsigned long long)mask) - 1;
       mask &= mask - 1;
-      const uint64_t k = __shfl(x.k, j);
-      const uint32_t n = __shfl(x.n, j);
-      const int32_t c = __shfl(x.c, j);
+      // the hit of lane j, read into scalar registers (j is wave-uniform)
+      const uint64_t k = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x.k >> 32), j) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x.k, j);
+      const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)x.n, j);
+      const int32_t c = __builtin_amdgcn_readlane((int)x.c, j);
       const bool keep = no_small_factor(k, n, c, small_primes, nsmall);
       if (keep && (int)lane == j) {
         const unsigned long long idx = atomicAdd(kept_count, 1ull);

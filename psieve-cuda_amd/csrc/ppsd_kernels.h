@@ -8,6 +8,16 @@ namespace ppsd {
 
 // One raw hit: p | k*2^n + c, with k odd in [kmin,kmax] and nmin <= n < nmax.
 // Same field order and widths as ppsd_factor in include/ppsd.h.
+// The constants of one small prime's arithmetic for the factor filter,
+// computed on the host.  The device table is padded to whole passes of 64
+// with copies of its last prime (a repeated divisor changes no verdict).
+struct SmallPrime {
+  uint32_t q;
+  uint32_t mq;   // floor(2^32 / q)
+  uint32_t md;   // floor(2^32 / (q - 1))
+  uint32_t c32;  // 2^32 mod q
+};
+
 struct Hit {
   uint64_t p;
   uint64_t k;
@@ -37,8 +47,8 @@ struct FilterArgs {
   uint64_t kmin, kmax;  // unused by the rule itself, kept for symmetry
   uint32_t nmin, nmax;
   uint32_t kstep, koffset;
-  const uint32_t *small_primes;  // odd primes 11 <= q < 32768
-  uint32_t nsmall;
+  const SmallPrime *small_primes;  // odd primes 11 <= q < 32768, padded
+  uint32_t nsmall;                 // a multiple of 64
   Hit *kept;
   unsigned long long *kept_count;
   uint64_t kept_cap;
