@@ -163,6 +163,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->b_next);
   (void)hipFree(c->b_hits);
   (void)hipFree(c->b_kept);
+  (void)hipFree(c->b_out);
   (void)hipFree(c->b_counts);
   if (c->h_pieces) (void)hipHostFree(c->h_pieces);
   if (c->h_tasks) (void)hipHostFree(c->h_tasks);
@@ -720,6 +721,8 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipMalloc(&c->b_hits, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_kept, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_counts, 2 * sizeof(unsigned long long)));
+  c->b_out_cap = c->b_hit_alloc;
+  HIP_TRY(hipMalloc(&c->b_out, c->b_out_cap * sizeof(BHit)));
   HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
@@ -844,6 +847,17 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
 
 // Run tasks [0, n) of h_tasks through the round graph; re-run halves when
 // the raw ring overflows.  Appends kept (or raw) hits to b_result.
+int ppsd::batch_flush(ppsd_ctx *c) {
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->b_out_n) {
+    const size_t old = c->b_result.size();
+    c->b_result.resize(old + c->b_out_n);
+    HIP_TRY(hipMemcpy(c->b_result.data() + old, c->b_out, c->b_out_n * sizeof(BHit), hipMemcpyDeviceToHost));
+    c->b_out_n = 0;
+  }
+  return 0;
+}
+
 static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int buf) {
   std::vector<std::pair<uint32_t, uint32_t>> todo{{0, n}};
   while (!todo.empty()) {
@@ -875,12 +889,14 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int b
       todo.push_back({f, k / 2});
       continue;
     }
+    // the results stay on the device until the batch ends or its output fills
     const uint64_t take = raw ? cnt[0] : cnt[1];
-    const size_t old = c->b_result.size();
-    c->b_result.resize(old + take);
-    if (take)
-      HIP_TRY(hipMemcpy(c->b_result.data() + old, raw ? c->b_hits : c->b_kept, take * sizeof(BHit),
-                        hipMemcpyDeviceToHost));
+    if (c->b_out_n + take > c->b_out_cap) {
+      int frc = batch_flush(c);
+      if (frc) return frc;
+    }
+    HIP_TRY(launch_emit(raw ? c->b_hits : c->b_kept, take, c->b_pieces, c->b_out + c->b_out_n, c->stream));
+    c->b_out_n += take;
     c->st.raw_hits += cnt[0];
     c->st.factors += cnt[1];
   }
@@ -926,8 +942,8 @@ int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, s
 // round's task queue on c->stream.  Returns 1 (no error, nothing recorded)
 // when the pieces overflowed the pool and the caller must split them.
 int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
-                     const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit, int buf,
-                     hipEvent_t ev0, hipEvent_t ev1) {
+                     const std::vector<const ppsd_params *> &par, bool raw, int buf, hipEvent_t ev0,
+                     hipEvent_t ev1) {
   const uint32_t np = (uint32_t)(e - b);
   HIP_TRY(hipEventSynchronize(ev1));
   c->st.sieve_ms += elapsed(ev0, ev1);
@@ -951,7 +967,6 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     D.nprimes = off[i + 1] - off[i];
     D.unit = P.unit;
     D.pre = choose_form(u, c->b_r, c->b_u, c->pre_mode, false).pre;
-    piece_unit[i] = P.unit;
     c->b_unit_primes[P.unit] += D.nprimes;
     c->b_unit_sums[P.unit] += sums[i + 1] - sums[i];
     c->st.primes += D.nprimes;
@@ -968,7 +983,6 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     hpw[i] = D.nwin ? hits_per_prime(*par[pieces[b + i].unit], pieces[b + i].lo | 1) / D.nwin : 0.0;
   }
   const double room = 0.5 * (double)c->b_hit_cap;
-  const size_t first_hit = c->b_result.size();
   for (size_t t = 0; t < tasks.size();) {
     size_t k = 0;
     double e = 0;
@@ -983,17 +997,7 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     if (rc) return rc;
     t += k;
   }
-  // piece index -> caller's unit
-  for (size_t h = first_hit; h < c->b_result.size(); ++h) c->b_result[h].piece = piece_unit[c->b_result[h].piece];
   return 0;
-}
-
-// One round on c->stream and pool 0: sieve, then walk (1 = pool overflow).
-int ppsd::run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
-                    const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit) {
-  const int rc = sieve_round(c, pieces, b, e, 0, c->stream, c->ev[2], c->ev[3]);
-  if (rc) return rc;
-  return walk_round(c, pieces, b, e, par, raw, piece_unit, 0, c->ev[2], c->ev[3]);
 }
 
 int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int raw) {
@@ -1020,6 +1024,7 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
     if (rc) return rc;
   }
   c->b_result.clear();
+  c->b_out_n = 0;
   c->b_unit_primes.assign(nunits, 0);
   c->b_unit_sums.assign(nunits, 0);
   c->b_nunits = nunits;
@@ -1048,7 +1053,6 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   // Round i+1 is sieved on stream2 into the other pool while round i walks
   // on the context's stream; round i+1's sieve is enqueued only after round
   // i-1's walk (which read that pool) has finished.
-  std::vector<uint32_t> piece_unit(c->b_piece_cap);
   auto sieve = [&](size_t i) {
     const int buf = (int)(i & 1);
     return sieve_round(c, pieces, rounds[i].first, rounds[i].second, buf, c->stream2, c->ev2[buf][0],
@@ -1066,21 +1070,21 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
       rc = sieve(i + 1);
       if (rc) return rc;
     }
-    rc = walk_round(c, pieces, b, e, par, raw != 0, piece_unit, buf, c->ev2[buf][0], c->ev2[buf][1]);
+    rc = walk_round(c, pieces, b, e, par, raw != 0, buf, c->ev2[buf][0], c->ev2[buf][1]);
     if (rc == 1) {  // pool overflow (rare): this round's pieces again, in two halves, on pool `buf`
       if (e - b == 1) return set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
       HIP_TRY(hipStreamSynchronize(c->stream2));  // the sieve buffers are shared with round i+1's sieve
       const size_t mid = b + (e - b) / 2;
       for (const auto &[x, y] : {std::make_pair(b, mid), std::make_pair(mid, e)}) {
         rc = sieve_round(c, pieces, x, y, buf, c->stream, c->ev[2], c->ev[3]);
-        if (!rc) rc = walk_round(c, pieces, x, y, par, raw != 0, piece_unit, buf, c->ev[2], c->ev[3]);
+        if (!rc) rc = walk_round(c, pieces, x, y, par, raw != 0, buf, c->ev[2], c->ev[3]);
         if (rc) return rc < 0 ? rc : set_err(PPSD_ERR_FAILURE, "prime pool overflow");
       }
     } else if (rc) {
       return rc;
     }
   }
-  return 0;
+  return batch_flush(c);
 }
 
 int ppsd_batch_results(ppsd_ctx *c, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out) {

@@ -120,6 +120,17 @@ __global__ void snapshot_kernel(const uint32_t *count, const unsigned long long 
   }
 }
 
+// Appends n hits of a ring to the batch's output on the device, with the
+// round-local piece index of each replaced by the caller's unit index.
+__global__ __launch_bounds__(256) void emit_kernel(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    BHit h = src[i];
+    h.piece = pieces[h.piece].unit;
+    dst[i] = h;
+  }
+}
+
 template <int R, int U>
 static hipError_t launch_persist_u(const BatchArgs &a, uint32_t grid, hipStream_t st) {
   hipLaunchKernelGGL((persist_walk_kernel<R, U>), dim3(grid), dim3(256), 0, st, a);
@@ -141,6 +152,12 @@ hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipSt
 
 hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st) {
   hipLaunchKernelGGL(batch_filter_kernel, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst, hipStream_t st) {
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+  if (blocks) hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, src, n, pieces, dst);
   return hipGetLastError();
 }
 

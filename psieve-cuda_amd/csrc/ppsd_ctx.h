@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ppsd.h"
@@ -24,6 +25,27 @@ uint64_t isqrt64(uint64_t x);
 std::vector<uint32_t> host_primes(uint32_t lim);
 
 enum class Pending { none, primes, range };
+
+// An allocator whose construct() without arguments default-initialises, so
+// that resize() before a bulk copy does not zero the new elements first.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept {
+    ::new ((void *)p) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
 
 // A p-subrange of one unit of a batched search.
 struct Piece {
@@ -116,12 +138,18 @@ struct ppsd_ctx {
   int b_graph_key[2] = {-1, -1};
   int b_r = 1, b_u = 8, b_waves = 8;
   uint32_t b_seg_min = 2048, b_seg_max = 65536;
-  std::vector<BHit> b_result;
+  BHit *b_out = nullptr;  // the batch's results so far, on the device, unit-indexed
+  uint64_t b_out_cap = 0, b_out_n = 0;
+  std::vector<BHit, ppsd::NoInitAlloc<BHit>> b_result;  // results copied back (batch_flush)
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;
 };
 
 namespace ppsd {
+
+// Copies the device output of the batch so far (b_out) to b_result, after
+// the context's stream has finished.
+int batch_flush(ppsd_ctx *c);
 
 // What the walk needs of a unit's parameters.
 bool walk_domain_ok(const ppsd_params *p);
@@ -147,22 +175,18 @@ WalkForm choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u);
 // Milliseconds between two recorded events (0 on error).
 float elapsed(hipEvent_t a, hipEvent_t b);
 // Batched walk: buffers (allocated on first use), sieving primes for every
-// p < pmax, and one round over pieces [b, e) of a piece list.  run_round
-// appends the round's hits to c->b_result tagged with piece_unit[] of their
-// piece, adds to c->b_unit_primes / b_unit_sums, and returns 1 (no error)
-// when the pieces overflowed the prime pool (nothing recorded then).
+// p < pmax, and the two halves of a round over pieces [b, e) of a piece
+// list: enqueue the sieve of the pieces into pool `buf` on stream st
+// (ev0/ev1 recorded around it, ev1 after its offsets reached the host);
+// then wait for ev1 and walk on c->stream.  walk_round appends the round's
+// hits to the device output b_out tagged with their piece's unit (see
+// batch_flush), adds to c->b_unit_primes / b_unit_sums, and returns 1 (no
+// error) when the pieces overflowed the prime pool (nothing recorded then).
 int batch_alloc(ppsd_ctx *c);
 int ensure_sieving(ppsd_ctx *c, uint64_t pmax);
-int run_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
-              const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit);
-// The two halves of a round, for callers that overlap them: enqueue the
-// sieve of pieces [b, e) into pool `buf` on stream st (ev0/ev1 recorded
-// around it, ev1 after its offsets reached the host); then wait for ev1 and
-// walk on c->stream (1 = the pool overflowed, nothing recorded).
 int sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e, int buf, hipStream_t st,
                 hipEvent_t ev0, hipEvent_t ev1);
 int walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e,
-               const std::vector<const ppsd_params *> &par, bool raw, std::vector<uint32_t> &piece_unit, int buf,
-               hipEvent_t ev0, hipEvent_t ev1);
+               const std::vector<const ppsd_params *> &par, bool raw, int buf, hipEvent_t ev0, hipEvent_t ev1);
 
 }  // namespace ppsd

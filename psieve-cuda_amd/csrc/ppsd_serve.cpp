@@ -148,27 +148,29 @@ struct Slice {
 };
 
 // Walk a sieved slice and hand its results to its requests.
-void finish_slice(ppsd_server *s, Slice &sl, std::vector<uint32_t> &piece_unit) {
+void finish_slice(ppsd_server *s, Slice &sl) {
   ppsd_ctx *c = s->ctx;
   const size_t np = sl.pieces.size();
   const double t0 = s->now();
   c->b_result.clear();
+  c->b_out_n = 0;
   c->b_unit_primes.assign(np, 0);
   c->b_unit_sums.assign(np, 0);
   int rc = sl.rc;
-  if (!rc) rc = walk_round(c, sl.pieces, 0, np, sl.par, s->cfg.raw != 0, piece_unit, sl.buf, c->ev2[sl.buf][0],
+  if (!rc) rc = walk_round(c, sl.pieces, 0, np, sl.par, s->cfg.raw != 0, sl.buf, c->ev2[sl.buf][0],
                            c->ev2[sl.buf][1]);
   if (rc == 1 && np > 1) {  // more primes than the pool holds (rare): the pieces in two halves
     rc = (int)hipStreamSynchronize(c->stream2) ? set_err(PPSD_ERR_FAILURE, "stream wait failed") : 0;
     const size_t mid = np / 2;
     for (const auto &[x, y] : {std::make_pair((size_t)0, mid), std::make_pair(mid, np)}) {
       if (!rc) rc = sieve_round(c, sl.pieces, x, y, sl.buf, c->stream, c->ev[2], c->ev[3]);
-      if (!rc) rc = walk_round(c, sl.pieces, x, y, sl.par, s->cfg.raw != 0, piece_unit, sl.buf, c->ev[2], c->ev[3]);
+      if (!rc) rc = walk_round(c, sl.pieces, x, y, sl.par, s->cfg.raw != 0, sl.buf, c->ev[2], c->ev[3]);
       if (rc == 1) rc = set_err(PPSD_ERR_FAILURE, "prime pool overflow in a slice");
     }
   } else if (rc == 1) {
     rc = set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
   }
+  if (!rc) rc = batch_flush(c);
   const double dt = s->now() - t0;
   double steps = 0;
   {
@@ -217,7 +219,6 @@ void serve_loop(ppsd_server *s) {
   ppsd_ctx *c = s->ctx;
   (void)hipSetDevice(c->device);
   std::vector<Request *> act;
-  std::vector<uint32_t> piece_unit(c->b_piece_cap);
   Slice slice[2];
   int nxt = 0;  // the slice being planned; slice[nxt ^ 1] is the one to walk
   for (;;) {
@@ -252,7 +253,7 @@ void serve_loop(ppsd_server *s) {
       next.rc = sieve_round(c, next.pieces, 0, np, next.buf, c->stream2, c->ev2[next.buf][0], c->ev2[next.buf][1]);
       next.live = true;
     }
-    if (cur.live) finish_slice(s, cur, piece_unit);
+    if (cur.live) finish_slice(s, cur);
     nxt ^= 1;
   }
 }
