@@ -26,7 +26,9 @@ import ppsd_units  # noqa: E402
 CACHE_DIR = os.path.join(os.path.dirname(HERE), "tuning")
 
 PLAIN_GRID = dict(r=(1, 2), u=(4, 8, 16), wg=(64, 128, 256))
-BATCH_GRID = dict(r=(1, 2), u=(4, 8, 16), waves=(4, 8, 16), seg=((2048, 65536), (8192, 262144)))
+# waves per SIMD below the 8 a SIMD holds leave room for the next round's
+# sieve, which runs beside the walk on a second stream
+BATCH_GRID = dict(r=(1, 2), u=(4, 8, 16), waves=(6, 7, 8), seg=((2048, 65536),))
 
 # the plain sample: one batch of the 2^50 workload (k <= 1e4, n <= 6e6)
 PLAIN_SAMPLE = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000)
@@ -88,7 +90,7 @@ def tune_plain(device: int = 0, reps: int = 2, log=None):
     return {"best": {k: top[k] for k in ("r", "u", "wg")}, "trials": rows}
 
 
-def tune_batch(device: int = 0, units: int = 64, log=None):
+def tune_batch(device: int = 0, units: int = 256, log=None):
     """Fastest (r, u, waves per SIMD, segment lengths) for ppsd_batch_run on
     the first `units` units of the seeded mixed workload."""
     params = [u.params for u in ppsd_units.mix_units(units)]
@@ -129,7 +131,7 @@ def main():
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=CACHE_DIR, help="directory of the cache file")
-    ap.add_argument("--units", type=int, default=64, help="mixed-workload units timed per batched shape")
+    ap.add_argument("--units", type=int, default=256, help="mixed-workload units timed per batched shape")
     a = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     entry = {"device": device_key(0), "plain": tune_plain(log=log), "batch": tune_batch(units=a.units, log=log)}
