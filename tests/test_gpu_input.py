@@ -8,7 +8,7 @@ import subprocess
 import pytest
 
 import pps_oracle as orc
-from conftest import REPO
+from conftest import REPO, read_lines
 from sievefiles import candidates, write_abcd, write_newpgen
 
 pytestmark = pytest.mark.gpu
@@ -53,3 +53,34 @@ def test_sieve_file(tmp_path, fmt, sign):
         d3 = tmp_path / "ref"
         d3.mkdir()
         assert run(REF, [f"-p{LO}", f"-P{HI}", "-i", path, "-q"], str(d3)) == want
+
+
+def _fixture_run(tmp_path, name, p0, p1, fmt):
+    """A sieve file holding exactly the (k, n) of a reference golden's lines,
+    searched over the golden's p-range: every factor of those terms in the
+    range is a line of the golden (the reference's run listed them all), so
+    the factors written must be exactly the golden."""
+    want = {orc.parse_line(l) for l in read_lines(name)}
+    terms = sorted({(k, n) for _, k, n, _ in want})
+    path = os.path.join(tmp_path, "sieve.txt")
+    (write_abcd if fmt == "abcd" else write_newpgen)(path, terms, 1)
+    for exe, sub in ((EXE, "ours"), (REF, "ref")):
+        if not os.path.exists(exe):
+            continue
+        d = tmp_path / sub
+        d.mkdir()
+        assert run(exe, [f"-p{p0}", f"-P{p1}", "-i", path, "-q"], str(d)) == want
+
+
+def test_big_p_reference_golden(tmp_path):
+    """pps/big_p_test.sh: -p80T -P80000050M over a sieve file, golden
+    pps/pps_test80t.txt (8 lines).  The script's sieve file
+    (pps/ppse_37TE1.txt) is not in the tree, so the file here lists just
+    the golden's terms."""
+    _fixture_run(tmp_path, "pps_test80t.txt", "80T", "80000050M", "newpgen")
+
+
+def test_ok_reference_golden(tmp_path):
+    """pps/OKsort.txt (15 lines) over the many_n range, from an ABCD file of
+    its terms."""
+    _fixture_run(tmp_path, "OKsort.txt", "42070e9", "42070050e6", "abcd")

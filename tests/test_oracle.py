@@ -282,6 +282,29 @@ def test_many_n_oracle_slice():
     assert cnt == 63668
 
 
+def test_ok_fixture_is_subset_of_filtered():
+    """pps/OKsort.txt (= pps/pps_test_ok.txt): what a run over the many_n
+    range with a sieve file wrote; every line is in the oracle's filtered
+    divisor set of that range."""
+    ok = read_lines("OKsort.txt")
+    assert len(ok) == 15
+    assert set(ok) <= set(read_lines("many_n_filtered.txt"))
+
+
+def test_big_p_fixture_lines_kept():
+    """pps/pps_test80t.txt (the golden of pps/big_p_test.sh, p in [80T,
+    80000050M)): each line is a prime divisor that the oracle's test_factor
+    keeps."""
+    lines = read_lines("pps_test80t.txt")
+    assert len(lines) == 8
+    pr = orc.derive(pmin=80 * 10**12, pmax=80000050 * 10**6, kmin=1241, kmax=9209, nmin=50848, nmax=1487243)
+    for l in lines:
+        p, k, n, c = orc.parse_line(l)
+        assert 80 * 10**12 <= p < 80000050 * 10**6 and sympy.isprime(p)
+        assert (k * pow(2, n, p) + c) % p == 0
+        assert orc.lib().orc_test_factor(pr, k, n, c) == 1
+
+
 def test_negative_fixtures_rejected():
     """Lines of pps/newsort.txt and pps/test.txt that are not divisors."""
     lines = read_lines("negative_lines.txt")
