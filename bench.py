@@ -102,13 +102,18 @@ def mixed_units(n):
     st = ctx.stats()
     ctx.close()
     tp, parts, same_counts = 0.0, [], True
+    last = time.perf_counter()
     for i, p in enumerate(params):
         with ppsd.Context(p) as c:
+            ppsd_tune.apply_plain(c)  # each path with its own tuned shape
             t0 = time.perf_counter()
             f, cn, ps = c.run_range(p.pmin, p.pmax)
             tp += time.perf_counter() - t0
         same_counts &= (cn, ps) == (int(cnt[i]), int(sums[i]))
         parts.append((np.full(len(f), i, np.uint32), f))
+        if time.perf_counter() - last > 20:
+            print(f"mixed units, plain path: {i + 1}/{n} searched", file=sys.stderr, flush=True)
+            last = time.perf_counter()
 
     def canon(u, f):  # rows (unit, p, k, n, c) in lexicographic order
         order = np.lexsort((f["c"], f["n"], f["k"], f["p"], u))
