@@ -23,8 +23,10 @@
 #include <sys/resource.h>
 #include <sys/time.h>
 #include <time.h>
+#include <unistd.h>
 
 #include "pps_app.h"
+#include "pps_hostsieve.h"
 
 #define MAX_THREADS 32
 #define PMIN_MIN 3
@@ -42,7 +44,7 @@ static unsigned int report_opt = 60;       /* main.h:44 */
 static unsigned int checkpoint_opt = 300;  /* main.h:45 */
 static unsigned int priority_opt = 0;
 static unsigned int blocksize_opt = 524288, chunksize_opt = 16384, blocks_opt = 2; /* accepted, unused */
-static uint64_t chunk_span = UINT64_C(1) << 28;
+static uint64_t chunk_span = 0; /* --chunk; 0: 2^28, or 2^30 with --host-primes */
 static int host_primes = 0;
 static int help_opt = 0;
 
@@ -169,7 +171,7 @@ static void help(void) {
   printf("-t --nthreads=N    Start N threads on N GPUs (default N=1)\n");
   printf("-z --priority=N    Set process priority to nice N or {idle,low,normal}\n");
   printf("   --host-primes   Sieve primes on the host and pass them in batches\n");
-  printf("   --chunk=N       p-range handed to a worker at a time (default 2^28)\n");
+  printf("   --chunk=N       p-range handed to a worker at a time (default 2^28, 2^30 with --host-primes)\n");
   printf("-h --help          Print this help\n");
 }
 
@@ -389,12 +391,33 @@ static uint32_t *host_sieve_primes(uint32_t lim, uint32_t *n) {
 static uint32_t *hq;
 static uint32_t nhq;
 
+/* --host-primes: primes of a chunk gathered into batches of cthread_count
+ * (main.c:736-832) and handed to the application as they fill */
+struct batcher {
+  int th;
+  uint64_t *P, *lastP;
+  unsigned int plen, cap;
+  uint64_t count, sum;
+};
+
+static void batch_primes(void *arg, const uint64_t *p, size_t n) {
+  struct batcher *b = arg;
+  for (size_t i = 0; i < n; i++) {
+    b->count++;
+    b->sum += p[i];
+    b->P[b->plen++] = p[i];
+    if (b->plen == b->cap) {
+      app_thread_fun(b->th, b->P, b->lastP, b->cap);
+      b->plen = 0;
+    }
+  }
+}
+
 static void *thread_fun(void *arg) {
   const int th = (int)(long)arg;
   unsigned int cthread_count = app_thread_init(th);
   uint64_t *P = NULL, *lastP = NULL;
-  uint8_t *seg = NULL;
-  const uint64_t SEG = UINT64_C(1) << 20;
+  pps_hostsieve *hs = NULL;
   fprintf(stderr, "%sThread %d starting\n", bmprefix(), th);
   if (cthread_count == 0) {
     pthread_mutex_lock(&mtx);
@@ -407,7 +430,14 @@ static void *thread_fun(void *arg) {
   if (host_primes) {
     P = malloc(sizeof(uint64_t) * cthread_count);
     lastP = malloc(sizeof(uint64_t) * cthread_count);
-    seg = malloc(SEG);
+    /* the host's cores shared among the GPU workers, at most 16 sieve threads each */
+    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    long per = (ncpu > 0 ? ncpu : 1) / (num_threads > 0 ? num_threads : 1);
+    hs = pps_hostsieve_new(hq, nhq, (int)(per < 1 ? 1 : per > 16 ? 16 : per));
+    if (!P || !lastP || !hs) {
+      fprintf(stderr, "%sThread %d: out of memory for the host sieve\n", bmprefix(), th);
+      exit(EXIT_FAILURE);
+    }
   }
   for (chunk_t *c; (c = take_chunk()) != NULL;) {
     uint64_t count = 0, sum = 0;
@@ -422,41 +452,21 @@ static void *thread_fun(void *arg) {
         break;
       }
     } else {
-      /* main.c:736-832: odd candidates of the chunk, batches of cthread_count */
-      unsigned int plen = 0;
-      for (uint64_t base = c->p0 | 1; base < c->p1; base += 2 * SEG) {
-        uint64_t cnt = (c->p1 - base + 1) / 2;
-        if (cnt > SEG) cnt = SEG;
-        memset(seg, 0, cnt);
-        for (uint32_t i = 0; i < nhq; i++) {
-          uint64_t q = hq[i], first, r;
-          if (q * q >= base + 2 * cnt) break;
-          r = base % q;
-          first = r ? base + (q - r) : base;
-          if (!(first & 1)) first += q;
-          if (first < q * q) first = q * q;
-          for (uint64_t m = first; m < base + 2 * cnt; m += 2 * q) seg[(m - base) / 2] = 1;
-        }
-        for (uint64_t j = 0; j < cnt; j++)
-          if (!seg[j]) {
-            const uint64_t p = base + 2 * j;
-            count++;
-            sum += p;
-            P[plen++] = p;
-            if (plen == cthread_count) {
-              app_thread_fun(th, P, lastP, cthread_count);
-              plen = 0;
-            }
-          }
+      struct batcher b = {th, P, lastP, 0, cthread_count, 0, 0};
+      if (pps_hostsieve_range(hs, c->p0, c->p1, batch_primes, &b)) {
+        fprintf(stderr, "%sThread %d: out of memory in the host sieve\n", bmprefix(), th);
+        exit(EXIT_FAILURE);
       }
-      app_thread_fun1(th, P, lastP, cthread_count, plen);
+      app_thread_fun1(th, P, lastP, cthread_count, b.plen);
+      count = b.count;
+      sum = b.sum;
     }
     finish_chunk(c, count, sum);
   }
   app_thread_fini(th);
   free(P);
   free(lastP);
-  free(seg);
+  pps_hostsieve_free(hs);
   pthread_mutex_lock(&mtx);
   live_workers--;
   pthread_cond_broadcast(&cond);
@@ -528,6 +538,8 @@ int main(int argc, char *argv[]) {
   }
 
   /* chunks [pstart, pmax) */
+  /* host sieving spreads a chunk over several threads, which pays on longer chunks */
+  if (!chunk_span) chunk_span = host_primes ? UINT64_C(1) << 30 : UINT64_C(1) << 28;
   nchunks = (size_t)((pmax - pstart + chunk_span - 1) / chunk_span);
   if (pstart >= pmax) nchunks = 0;
   chunks = calloc(nchunks + 1, sizeof(chunk_t));
