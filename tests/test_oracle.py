@@ -220,6 +220,22 @@ def test_try_all_factors_vs_reference_driver():
             assert out == f"sm {q} | {k}*2^{n}+1"
 
 
+@pytest.mark.skipif(not os.path.exists(orc.REF_FACTOR_PROTH), reason="the reference's factor_proth driver is built only where its sources are")
+def test_golden_factors_reverify_in_reference_factor_proth():
+    """Every +1 line of the golden factor sets -- which the device's output
+    must equal, line for line -- passes the reference's own factor_proth.c
+    check: its stdin driver echoes a line with no odd factor below 2^15
+    unchanged (pps/factor_proth.c:255-282; the driver handles +1 only)."""
+    lines = []
+    for name in ("cfg1_factors.txt", "cfg2_slice.txt", "cfg3_slice.txt", "many_n_filtered.txt", "OKsort.txt",
+                 "pps_test80t.txt"):
+        lines += [l for l in read_lines(name) if l.endswith("+1")]
+    out = subprocess.run([orc.REF_FACTOR_PROTH], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                         check=True, timeout=120).stdout.strip().split("\n")
+    assert len(lines) > 450
+    assert out == lines
+
+
 def test_test_factor_rules():
     pr = orc.derive(pmin=10**12, kmin=3, kmax=9999, nmin=100, nmax=1000)
     rng = random.Random(3)
