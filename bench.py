@@ -129,6 +129,64 @@ def mixed_units(n):
             "batch_shape": shape}
 
 
+def _is_prime64(n):
+    """Deterministic Miller-Rabin for n < 3.3e24 (the first 12 prime bases)."""
+    if n < 2:
+        return False
+    small = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
+    for q in small:
+        if n % q == 0:
+            return n == q
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d, r = d // 2, r + 1
+    for a in small:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def verify_factors(f, kmin, kmax, nmin, nmax):
+    """Every factor line is valid: p prime, p | k*2^n+1, k odd in range, n in
+    range, and no odd prime q < 2^15 divides k*2^n+1 (the rule the filter
+    applies), the last by vectorised trial division."""
+    import numpy as np
+
+    p = [int(x) for x in f["p"]]
+    k = [int(x) for x in f["k"]]
+    n = [int(x) for x in f["n"]]
+    ok = all(_is_prime64(pi) and (ki * pow(2, ni, pi) + 1) % pi == 0 and ki % 2 == 1 and kmin <= ki <= kmax
+             and nmin <= ni < nmax for pi, ki, ni in zip(p, k, n))
+    if not ok or not p:
+        return ok
+    sieve = np.ones(32768, dtype=bool)
+    sieve[:2] = False
+    for q in range(2, 182):
+        if sieve[q]:
+            sieve[q * q::q] = False
+    ka = np.array(k, dtype=np.uint64)
+    na = np.array(n, dtype=np.int64)
+    for q in np.nonzero(sieve)[0][1:]:  # odd primes
+        q = int(q)
+        e = na % (q - 1)  # Fermat
+        t = np.ones(len(na), dtype=np.int64)
+        b = np.full(len(na), 2, dtype=np.int64)
+        while e.any():
+            t = np.where(e & 1, t * b % q, t)
+            b = b * b % q
+            e >>= 1
+        if np.any(((ka % np.uint64(q)).astype(np.int64) * t + 1) % q == 0):
+            return False
+    return True
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -156,12 +214,13 @@ def main():
     if full:
         p = WORKLOAD["pmin"]  # the whole range, warm-up batches included
     p_start = p
-    factors, nsteps, count, psum = 0, 0, 0, 0
+    factors, nsteps, count, psum, found = 0, 0, 0, 0, []
     t0 = time.perf_counter()
     while p < WORKLOAD["pmax"] and (full or nsteps < a.steps):
         p = ctx.submit_range(p, WORKLOAD["pmax"])
         f, c, s = ctx.collect()
         factors += len(f)
+        found.append(f)
         count += c
         psum = (psum + s) & (2**64 - 1)
         nsteps += 1
@@ -177,6 +236,10 @@ def main():
         checksum = {"count": count, "sum": f"0x{psum:016x}",
                     "matches_independent_sieve": (count, psum) == (g["count"], g["sum"])}
 
+    import numpy as np
+
+    valid = verify_factors(np.concatenate(found) if found else np.zeros(0, dtype=ppsd.FACTOR_DTYPE),
+                           params.kmin, params.kmax, params.nmin, params.nmax)
     value = st.primes / dt
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
     t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
@@ -204,6 +267,7 @@ def main():
             "nstep": params.nstep,
             "window_steps": st.window_steps,
             "factors": factors,
+            "factors_valid": valid,
             "walk_ms": st.walk_ms,
             "sieve_ms": st.sieve_ms,
             "filter_ms": st.filter_ms,
