@@ -89,6 +89,7 @@ struct ppsd_ctx {
   uint32_t *d_sp = nullptr;     // sieving primes <= qmax
   uint32_t nsp = 0, nsp_small = 0;
   uint64_t qmax = 0;
+  uint64_t *d_spinv = nullptr;  // floor((2^64-1)/q) of the LDS-sieved primes d_sp[0..nsp_small)
   std::vector<uint32_t> sp_sample;  // d_sp[0], d_sp[4096], ... (batched walk)
   uint32_t *d_bitmap = nullptr;
   uint32_t *d_blockcount = nullptr;

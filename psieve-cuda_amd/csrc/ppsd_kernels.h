@@ -66,6 +66,7 @@ struct SieveArgs {
   uint32_t nq_small; // q[0..nq_small) are marked per segment in LDS
   uint32_t nq;       // q[nq_small..nq) are marked with global atomics
   uint32_t *bitmap;  // ceil(nbits/32) words; set bit = composite
+  const uint64_t *qinv;  // floor((2^64-1)/q) for q[0..nq_small), or null
 };
 
 // Segment geometry of the LDS sieve: 2^18 candidates (2^19 integers) per

@@ -142,11 +142,23 @@ __device__ __forceinline__ uint64_t first_multiple(uint64_t from, uint32_t q) {
   return m;
 }
 
+// The same, with the remainder from inv = floor((2^64-1)/q): the estimate
+// hi64(from*inv) is the quotient or one less, so one subtraction finishes.
+__device__ __forceinline__ uint64_t first_multiple_inv(uint64_t from, uint32_t q, uint64_t inv) {
+  const uint64_t qq = (uint64_t)q * q;
+  if (qq >= from) return qq;
+  uint64_t r = from - __umul64hi(from, inv) * q;
+  r -= r >= q ? q : 0;
+  uint64_t m = from + (r ? q - r : 0);
+  if ((m & 1) == 0) m += q;
+  return m;
+}
+
 // One workgroup per segment of kSegBits candidates.  Primes below 1024 are
 // marked cooperatively by the whole workgroup (each has many multiples in a
 // segment); the rest of q[0..nq_small) one prime per thread.  Every bitmap
 // word of the segment is written, tail bits past nbits set.
-__global__ __launch_bounds__(256) void sieve_small_kernel(SieveArgs a) {
+__global__ __launch_bounds__(1024) void sieve_small_kernel(SieveArgs a) {
   __shared__ uint32_t seg[kSegWords];
   const uint64_t b0 = (uint64_t)blockIdx.x * kSegBits;
   const uint64_t rem = a.nbits - b0;
@@ -157,7 +169,7 @@ __global__ __launch_bounds__(256) void sieve_small_kernel(SieveArgs a) {
   uint32_t i = 0;
   for (; i < a.nq_small && a.q[i] < 1024u; ++i) {
     const uint32_t q = a.q[i];
-    const uint64_t m = first_multiple(seg_lo, q);
+    const uint64_t m = a.qinv ? first_multiple_inv(seg_lo, q, a.qinv[i]) : first_multiple(seg_lo, q);
     const uint64_t off = (m - seg_lo) >> 1;
     if (off >= nb) continue;
     for (uint64_t b = off + (uint64_t)threadIdx.x * q; b < nb; b += (uint64_t)blockDim.x * q)
@@ -165,7 +177,7 @@ __global__ __launch_bounds__(256) void sieve_small_kernel(SieveArgs a) {
   }
   for (uint32_t t = i + threadIdx.x; t < a.nq_small; t += blockDim.x) {
     const uint32_t q = a.q[t];
-    const uint64_t m = first_multiple(seg_lo, q);
+    const uint64_t m = a.qinv ? first_multiple_inv(seg_lo, q, a.qinv[t]) : first_multiple(seg_lo, q);
     for (uint64_t b = (m - seg_lo) >> 1; b < nb; b += q) atomicOr(&seg[b >> 5], 1u << (b & 31));
   }
   __syncthreads();
@@ -192,7 +204,8 @@ __global__ __launch_bounds__(256) void sieve_large_kernel(SieveArgs a) {
 
 hipError_t launch_sieve(const SieveArgs &a, hipStream_t st) {
   const uint32_t nseg = (uint32_t)((a.nbits + kSegBits - 1) / kSegBits);
-  hipLaunchKernelGGL(sieve_small_kernel, dim3(nseg), dim3(256), 0, st, a);
+  // 1024 threads: a segment's block walks every LDS-sieved prime (up to 2^21)
+  hipLaunchKernelGGL(sieve_small_kernel, dim3(nseg), dim3(1024), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.nq > a.nq_small) {
