@@ -143,6 +143,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_bitmap);
   (void)hipFree(c->d_cand);
   (void)hipFree(c->d_blockcount);
+  (void)hipFree(c->d_blocksum);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (int i = 0; i < 2; ++i) {
     if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
@@ -270,6 +271,7 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   OPEN_TRY(hipMalloc(&c->d_kept, c->hit_cap * sizeof(Hit)));
   OPEN_TRY(hipMalloc(&c->d_bitmap, (c->nbits_max / 32 + 1) * sizeof(uint32_t)));
   OPEN_TRY(hipMalloc(&c->d_blockcount, (c->nbits_max / 32 / kScanBlock + 2) * sizeof(uint32_t)));
+  OPEN_TRY(hipMalloc(&c->d_blocksum, (c->nbits_max / 32 / kScanBlock + 2) * sizeof(unsigned long long)));
   OPEN_TRY(hipHostMalloc((void **)&c->h_stage, (size_t)c->batch * sizeof(uint64_t), hipHostMallocDefault));
   {
     std::vector<SmallPrime> sm;
@@ -511,7 +513,8 @@ static int enqueue_sieve(ppsd_ctx *c, uint64_t lo, uint64_t nbits) {
   sa.qinv = c->d_spinv;
   HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   HIP_TRY(launch_sieve(sa, c->stream));
-  HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_P, c->batch, c->d_count, c->d_counters + 2,
+  HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_blocksum, c->d_P, c->batch, c->d_count,
+                           c->d_counters + 2,
                            c->stream));
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   return 0;
@@ -941,7 +944,8 @@ int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, s
       sa.bitmap = c->d_bitmap;
       sa.qinv = c->d_spinv;
       HIP_TRY(launch_sieve(sa, st));
-      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->b_pool[buf], (uint32_t)c->b_pool_cap,
+      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_blocksum, c->b_pool[buf],
+                               (uint32_t)c->b_pool_cap,
                                c->b_pool_count[buf], c->b_pool_sum[buf], st));
       lo += 2 * nbits;
     }

@@ -93,6 +93,7 @@ struct ppsd_ctx {
   std::vector<uint32_t> sp_sample;  // d_sp[0], d_sp[4096], ... (batched walk)
   uint32_t *d_bitmap = nullptr;
   uint32_t *d_blockcount = nullptr;
+  unsigned long long *d_blocksum = nullptr;  // per compaction block: prime sum
   uint64_t nbits_max = 0;
   uint64_t *h_stage = nullptr;  // pinned upload staging
   uint8_t *d_cand = nullptr;    // candidate list of a sieve file (ppsd_set_candidates)

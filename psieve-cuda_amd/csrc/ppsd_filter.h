@@ -91,6 +91,20 @@ __device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t 
   return true;
 }
 
+// Slots for the lanes set in keep (the same mask in every lane) at the end
+// of a ring, in lane order, with one atomic per wave: a counter every kept
+// hit incremented on its own serialised the filter on hit-dense units.
+// Returns this lane's slot (meaningful only where its bit is set).
+__device__ __forceinline__ unsigned long long wave_slot(uint64_t keep, unsigned long long *count) {
+  const uint32_t lane = __lane_id();
+  unsigned long long base = 0;
+  if (keep) {
+    if (lane == (uint32_t)(__ffsll((unsigned long long)keep) - 1)) base = atomicAdd(count, (unsigned long long)__popcll(keep));
+    base = __shfl(base, __ffsll((unsigned long long)keep) - 1);
+  }
+  return base + (unsigned long long)__popcll(keep & ((1ull << lane) - 1));
+}
+
 // Filters raw hits [0, nraw) into a kept ring.  Each wave takes 64 hits at a
 // time, one per lane, applies the cheap part of the rule lane-parallel, then
 // runs the trial division wave-wide for each survivor in turn; a kept hit is
@@ -112,6 +126,7 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
     if (valid) rule(x, kstep, koffset, nmin, nmax);
     const bool cand = valid && rule_cheap(x.k, x.n, x.c, kstep, koffset, nmin, nmax);
     uint64_t mask = __ballot(cand);
+    uint64_t keepmask = 0;  // wave-uniform: each verdict below is the wave's
     while (mask) {
       const int j = __ffsll((unsigned long long)mask) - 1;
       mask &= mask - 1;
@@ -120,12 +135,10 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x.k, j);
       const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)x.n, j);
       const int32_t c = __builtin_amdgcn_readlane((int)x.c, j);
-      const bool keep = no_small_factor(k, n, c, small_primes, nsmall);
-      if (keep && (int)lane == j) {
-        const unsigned long long idx = atomicAdd(kept_count, 1ull);
-        if (idx < kept_cap) kept[idx] = x;
-      }
+      if (no_small_factor(k, n, c, small_primes, nsmall)) keepmask |= 1ull << j;
     }
+    const unsigned long long idx = wave_slot(keepmask, kept_count);
+    if (((keepmask >> lane) & 1) && idx < kept_cap) kept[idx] = x;
   }
 }
 

@@ -81,10 +81,11 @@ hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st);
 hipError_t launch_sieve(const SieveArgs &a, hipStream_t st);
 // Compaction: primes of the chunk to out (ascending), appended at *count_d;
 // at most cap entries of out are written, *count_d counts them all; *sum_d
-// accumulates their sum mod 2^64.
+// accumulates their sum mod 2^64 through blocksum (one entry per kScanBlock
+// words of the chunk).
 hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
-                            uint64_t *out, uint32_t cap, uint32_t *count_d, unsigned long long *sum_d,
-                            hipStream_t st);
+                            unsigned long long *blocksum, uint64_t *out, uint32_t cap, uint32_t *count_d,
+                            unsigned long long *sum_d, hipStream_t st);
 hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
                             uint32_t *out, uint32_t cap, uint32_t *count_d, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);

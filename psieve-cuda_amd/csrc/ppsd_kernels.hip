@@ -102,21 +102,28 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
 __global__ __launch_bounds__(256) void filter_cand_kernel(FilterArgs a) {
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
+  // wave-uniform loop (the stride is a whole number of waves), so that the
+  // kept lanes of a wave reserve their slots together
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; h < nraw; h += stride) {
-    const Hit x = a.hits[h];
-    const uint64_t b = x.k / a.kstep;
-    bool keep = x.k == (uint64_t)a.kstep * b + a.koffset && x.n >= a.nmin && x.n < a.nmax;
+  const uint64_t first = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u);
+  for (uint64_t h0 = first; h0 < nraw; h0 += stride) {
+    const uint64_t h = h0 + __lane_id();
+    Hit x{};
+    bool keep = h < nraw;
     if (keep) {
-      const uint64_t row = (uint64_t)x.n - a.cand_n0;
-      const uint64_t bit = b - a.cand_b0;
-      keep = x.n >= a.cand_n0 && row < a.cand_rows && b >= a.cand_b0 && bit < a.cand_row_bytes * 8 &&
-             ((a.cand[row * a.cand_row_bytes + bit / 8] >> (bit % 8)) & 1u);
+      x = a.hits[h];
+      const uint64_t b = x.k / a.kstep;
+      keep = x.k == (uint64_t)a.kstep * b + a.koffset && x.n >= a.nmin && x.n < a.nmax;
+      if (keep) {
+        const uint64_t row = (uint64_t)x.n - a.cand_n0;
+        const uint64_t bit = b - a.cand_b0;
+        keep = x.n >= a.cand_n0 && row < a.cand_rows && b >= a.cand_b0 && bit < a.cand_row_bytes * 8 &&
+               ((a.cand[row * a.cand_row_bytes + bit / 8] >> (bit % 8)) & 1u);
+      }
     }
-    if (keep) {
-      const unsigned long long idx = atomicAdd(a.kept_count, 1ull);
-      if (idx < a.kept_cap) a.kept[idx] = x;
-    }
+    const uint64_t keepmask = __ballot(keep);
+    const unsigned long long idx = wave_slot(keepmask, a.kept_count);
+    if (keep && idx < a.kept_cap) a.kept[idx] = x;
   }
 }
 
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(256) void scan_kernel(uint32_t *blockcount, uint32_
 template <typename T>
 __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, uint32_t nwords, uint64_t lo,
                                                       const uint32_t *blockoff, T *out, uint32_t cap,
-                                                      unsigned long long *sum_d) {
+                                                      unsigned long long *blocksum) {
   const uint32_t idx = blockIdx.x * kScanBlock + threadIdx.x;
   uint32_t w = idx < nwords ? ~bitmap[idx] : 0u;
   uint32_t total;
@@ -288,33 +295,55 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *bitmap, ui
     ++pos;
     s += x;
   }
-  if (sum_d) {
-    // wave reduction, then one atomic per wave
+  if (blocksum) {
+    // the block's prime sum, stored for sum_blocks_kernel (atomics on one
+    // address from every wave serialised this kernel)
+    __shared__ unsigned long long wsum[4];
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (__lane_id() == 0) atomicAdd(sum_d, s);
+    if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
+}
+
+// Adds the blocks' prime sums to *sum_d (one workgroup).
+__global__ __launch_bounds__(256) void sum_blocks_kernel(const unsigned long long *blocksum, uint32_t n,
+                                                         unsigned long long *sum_d) {
+  __shared__ unsigned long long part[256];
+  unsigned long long s = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += blocksum[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t h = 128; h >= 1; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *sum_d += part[0];
 }
 
 template <typename T>
 static hipError_t launch_compact(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, T *out,
-                                 uint32_t cap, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
+                                 uint32_t cap, uint32_t *count_d, unsigned long long *blocksum,
+                                 unsigned long long *sum_d, hipStream_t st) {
   const uint32_t nwords = (uint32_t)((nbits + 31) / 32);
   const uint32_t nblocks = (nwords + kScanBlock - 1) / kScanBlock;
   hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(256), 0, st, bitmap, nwords, blockcount);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, st, blockcount, nblocks, count_d);
   hipLaunchKernelGGL((compact_kernel<T>), dim3(nblocks), dim3(256), 0, st, bitmap, nwords, lo,
-                     (const uint32_t *)blockcount, out, cap, sum_d);
+                     (const uint32_t *)blockcount, out, cap, sum_d ? blocksum : nullptr);
+  if (sum_d) hipLaunchKernelGGL(sum_blocks_kernel, dim3(1), dim3(256), 0, st, blocksum, nblocks, sum_d);
   return hipGetLastError();
 }
 
-hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint64_t *out,
-                            uint32_t cap, uint32_t *count_d, unsigned long long *sum_d, hipStream_t st) {
-  return launch_compact<uint64_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, sum_d, st);
+hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount,
+                            unsigned long long *blocksum, uint64_t *out, uint32_t cap, uint32_t *count_d,
+                            unsigned long long *sum_d, hipStream_t st) {
+  return launch_compact<uint64_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, blocksum, sum_d, st);
 }
 
 hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint32_t *out,
                             uint32_t cap, uint32_t *count_d, hipStream_t st) {
-  return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, nullptr, st);
+  return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, nullptr, nullptr, st);
 }
 
 // ---------------------------------------------------------------------------
