@@ -132,11 +132,13 @@ int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
  * whose kernels pick among test macros at compile time: pps/appcu.cu:779-801).
  * 1: the odd part of the state is <= kmax (ctz, shift, compare); 2: the
  * state is <= kmax*2^(s-1) (or, both signs, >= p minus that) -- one compare
- * per sign, and a chunk re-walk whenever a merely small state fires it; 0
- * (default): 2 where kmax*2^(s-1)/pmin makes such re-walks rare enough to
- * pay, with a shorter chunk than the tuned u when that pays more, else 1.
- * Results never depend on it.  plain_form and plain_u receive the form (1 or
- * 2) and the chunk length this context's own walk uses. */
+ * per sign, and a chunk re-walk whenever a merely small state fires it; 3:
+ * the low 32 bits of the odd part, minimised over a chunk of windows, are <=
+ * kmax -- fires on about kmax/2^32 of all states, for kmax < 2^32 only (else
+ * 1 is used); 0 (default): the form, and a chunk
+ * no longer than the tuned u, that a cost model of their re-walks prices
+ * lowest.  Results never depend on it.  plain_form and plain_u receive the
+ * form (1, 2 or 3) and the chunk length this context's own walk uses. */
 int ppsd_set_pretest(ppsd_ctx *ctx, int mode);
 int ppsd_get_pretest(const ppsd_ctx *ctx, int *mode, int *plain_form, int *plain_u);
 

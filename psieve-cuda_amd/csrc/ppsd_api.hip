@@ -320,7 +320,7 @@ int ppsd_get_tuning(const ppsd_ctx *c, int *r, int *u, int *wg) {
 
 int ppsd_set_pretest(ppsd_ctx *c, int mode) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
-  if (mode < 0 || mode > 2) return set_err(PPSD_ERR_INVALID_PARAM, "pre-test mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) return set_err(PPSD_ERR_INVALID_PARAM, "pre-test mode must be 0, 1, 2 or 3");
   c->pre_mode = mode;
   return 0;
 }
@@ -329,7 +329,7 @@ int ppsd_get_pretest(const ppsd_ctx *c, int *mode, int *plain_form, int *plain_u
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
   const WalkForm form = choose_form(c->prm, c->tune_r, c->tune_u, c->pre_mode, true);
   if (mode) *mode = c->pre_mode;
-  if (plain_form) *plain_form = form.pre == PRE_RANGE ? 2 : 1;
+  if (plain_form) *plain_form = form.pre == PRE_RANGE ? 2 : form.pre == PRE_ODDLO ? 3 : 1;
   if (plain_u) *plain_u = form.u;
   return 0;
 }
@@ -461,18 +461,29 @@ int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
 }
 
 ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u) {
+  const bool oddlo_ok = (p.kmax >> 32) == 0;
   if (mode == 1) return {PRE_EXACT, u};
   if (mode == 2) return {PRE_RANGE, u};
+  if (mode == 3) return {oddlo_ok ? PRE_ODDLO : PRE_EXACT, u};
   // false alarms per lane-window (the smallest p of the unit fires most)
-  const double f = (p.sign == 0 ? 2.0 : 1.0) * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
-  const double base = p.sign == 0 ? 0.56 : 0.75;
+  const double signs = p.sign == 0 ? 2.0 : 1.0;
+  const double f_range = signs * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
+  const double f_odd = signs * ldexp((double)p.kmax + 1.0, -32);
+  const double base_range = p.sign == 0 ? 0.56 : 0.75;
+  const double base_odd = p.sign == 0 ? 0.89 : 0.93;
   WalkForm best{PRE_EXACT, u};
-  double best_cost = 0.97;  // the range form must win by a margin
+  double best_cost = 1.0;
   for (int cu = u, halvings = 0; cu >= 4 && (free_u || cu == u); cu /= 2, ++halvings) {
-    const double cost = base + 1.95 * f * 64.0 * r * cu + 0.02 * halvings;
-    if (cost < best_cost) {
-      best_cost = cost;
+    // the range form must win by a margin of 0.03, the odd-low form by 0.01
+    const double range = base_range + 1.95 * f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
+    const double odd = base_odd + 1.95 * f_odd * 64.0 * r * cu + 0.02 * halvings + 0.01;
+    if (range < best_cost) {
+      best_cost = range;
       best = {PRE_RANGE, cu};
+    }
+    if (oddlo_ok && odd < best_cost) {
+      best_cost = odd;
+      best = {PRE_ODDLO, cu};
     }
   }
   return best;

@@ -61,6 +61,18 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P,
   walk_windows<SIGN, R, U, S32, PRE>(L, S, n0, T.nw, rule, sink);
 }
 
+template <int R, int U, int PRE>
+__device__ __forceinline__ void run_form(const BatchArgs &a, const PieceDesc &P, const Task &T) {
+  const bool s32 = P.s == 32;
+  if (P.sign == 1) {
+    if (s32) run_task<1, R, U, true, PRE>(a, P, T); else run_task<1, R, U, false, PRE>(a, P, T);
+  } else if (P.sign == -1) {
+    if (s32) run_task<-1, R, U, true, PRE>(a, P, T); else run_task<-1, R, U, false, PRE>(a, P, T);
+  } else {
+    if (s32) run_task<0, R, U, true, PRE>(a, P, T); else run_task<0, R, U, false, PRE>(a, P, T);
+  }
+}
+
 // Every wave pulls tasks until the queue is empty; all waves reach the exit.
 template <int R, int U>
 __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
@@ -72,24 +84,9 @@ __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
     if (t >= ntasks) return;
     const Task T = a.tasks[t];
     const PieceDesc P = a.pieces[T.piece];
-    const bool s32 = P.s == 32;
-    if (P.pre == PRE_RANGE) {
-      if (P.sign == 1) {
-        if (s32) run_task<1, R, U, true, PRE_RANGE>(a, P, T); else run_task<1, R, U, false, PRE_RANGE>(a, P, T);
-      } else if (P.sign == -1) {
-        if (s32) run_task<-1, R, U, true, PRE_RANGE>(a, P, T); else run_task<-1, R, U, false, PRE_RANGE>(a, P, T);
-      } else {
-        if (s32) run_task<0, R, U, true, PRE_RANGE>(a, P, T); else run_task<0, R, U, false, PRE_RANGE>(a, P, T);
-      }
-    } else {
-      if (P.sign == 1) {
-        if (s32) run_task<1, R, U, true, PRE_EXACT>(a, P, T); else run_task<1, R, U, false, PRE_EXACT>(a, P, T);
-      } else if (P.sign == -1) {
-        if (s32) run_task<-1, R, U, true, PRE_EXACT>(a, P, T); else run_task<-1, R, U, false, PRE_EXACT>(a, P, T);
-      } else {
-        if (s32) run_task<0, R, U, true, PRE_EXACT>(a, P, T); else run_task<0, R, U, false, PRE_EXACT>(a, P, T);
-      }
-    }
+    if (P.pre == PRE_RANGE) run_form<R, U, PRE_RANGE>(a, P, T);
+    else if (P.pre == PRE_ODDLO) run_form<R, U, PRE_ODDLO>(a, P, T);
+    else run_form<R, U, PRE_EXACT>(a, P, T);
   }
 }
 

@@ -34,7 +34,7 @@ struct WalkArgs {
   uint32_t s;               // window, 1..32
   uint32_t nwin;            // windows to walk: ceil((nmax - nmin) / s)
   uint64_t kmin, kmax;
-  int pre;                           // PRE_EXACT or PRE_RANGE (ppsd_walk.h)
+  int pre;                           // PRE_EXACT, PRE_RANGE or PRE_ODDLO (ppsd_walk.h)
   Hit *hits;                         // raw-hit ring
   unsigned long long *hit_count;     // slots requested (may exceed cap)
   uint64_t hit_cap;

@@ -73,6 +73,11 @@ hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid,
   if (a.pre == PRE_RANGE)
     return a.s == 32 ? launch_walk_s<true, PRE_RANGE>(a, sign, r, u, grid, block, st)
                      : launch_walk_s<false, PRE_RANGE>(a, sign, r, u, grid, block, st);
+  if (a.pre == PRE_ODDLO) {
+    if (a.kmax >> 32) return hipErrorInvalidValue;
+    return a.s == 32 ? launch_walk_s<true, PRE_ODDLO>(a, sign, r, u, grid, block, st)
+                     : launch_walk_s<false, PRE_ODDLO>(a, sign, r, u, grid, block, st);
+  }
   return a.s == 32 ? launch_walk_s<true, PRE_EXACT>(a, sign, r, u, grid, block, st)
                    : launch_walk_s<false, PRE_EXACT>(a, sign, r, u, grid, block, st);
 }

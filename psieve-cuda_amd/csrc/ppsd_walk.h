@@ -19,8 +19,24 @@ struct WindowRule {
 // (both signs: or >= p - that), one compare per sign -- a necessary
 // condition that also fires on states that are merely small, which costs a
 // re-walk of the chunk; the host picks it when that is rare, i.e. when
-// kmax * 2^(s-1) is a tiny fraction of p.
-enum { PRE_EXACT = 0, PRE_RANGE = 1 };
+// kmax * 2^(s-1) is a tiny fraction of p.  PRE_ODDLO (kmax < 2^32 only): the
+// low word of the odd part, minimised over the chunk, is <= kmax -- see
+// odd_low.
+enum { PRE_EXACT = 0, PRE_RANGE = 1, PRE_ODDLO = 2 };
+
+// Low word of the odd part of K: v_ffbl_b32, then v_alignbit_b32 takes bits
+// i..i+31 of K.  A hit K = k*2^i (k odd, k <= kmax < 2^32, i < s <= 32)
+// gives exactly k, so "the chunk's minimum is <= kmax" is necessary for a
+// hit in the chunk; it also fires on about kmax/2^32 of other states (whose
+// odd part is large but ends in a small low word), and on some states with
+// a zero low word, which hold no hit (i >= 32 >= s).  Two instructions per
+// state and one v_min3_u32 per two states, against the exact form's three
+// instructions (v_ffbl_b32, 64-bit shift, 64-bit compare) per state: in a
+// stream of multiplies every VALU instruction takes one issue slot, so the
+// instruction count is what matters.
+__device__ __forceinline__ uint32_t odd_low(uint64_t K) {
+  return __builtin_amdgcn_alignbit(hi32(K), lo32(K), ffbl(lo32(K)));
+}
 
 // Raw-hit ring of the plain walk: 24-byte records.
 struct RingSink {
@@ -118,9 +134,32 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
   }
 }
 
+// Pre-test of W windows of R states each, stepping them: true when one of
+// them may hold a hit.
+template <int SIGN, int R, int W, bool S32, int PRE>
+__device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a) {
+  bool any = false;
+  uint32_t lowest = ~0u;
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (PRE == PRE_ODDLO) {
+        lowest = min(lowest, odd_low(S.K[r]));
+        if (SIGN == 0) lowest = min(lowest, odd_low(L.p[r] - S.K[r]));
+      } else {
+        any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
+      }
+      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
+    }
+  }
+  if (PRE == PRE_ODDLO) any = lowest <= (uint32_t)a.kmax;
+  return any;
+}
+
 // Walk nwin windows from window start n, states S.  Steps run in chunks of U
-// and a chunk only leaves the straight-line path when one of its pre-tests
-// fired (then it is walked again with exact tests).
+// and a chunk only leaves the straight-line path when its pre-test fired
+// (then it is walked again with exact tests).
 template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
 __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                              const WindowRule &a, const Sink &sink) {
@@ -128,25 +167,12 @@ __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<
   uint32_t w = 0;
   for (; w + U <= nwin; w += U) {
     const LaneStates<R> snap = S;
-    bool any = false;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
-        S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
-      }
-    }
-    if (any) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
+    if (test_and_step<SIGN, R, U, S32, PRE>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
     n += U * s;
   }
   for (; w < nwin; ++w, n += s) {
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < R; ++r) any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
-    if (any) rescan<SIGN, R, S32>(L, S, n, 1, a, sink);
-#pragma unroll
-    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], s);
+    const LaneStates<R> snap = S;
+    if (test_and_step<SIGN, R, 1, S32, PRE>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, 1, a, sink);
   }
 }
 

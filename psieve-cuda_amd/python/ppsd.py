@@ -257,7 +257,7 @@ class Context:
         _check(lib().ppsd_set_pretest(self._h, mode), "ppsd_set_pretest")
 
     def pretest(self):
-        """(mode set, form the plain walk uses: 1 exact / 2 range, its chunk u)."""
+        """(mode set, form the plain walk uses: 1 exact / 2 range / 3 odd low word, its chunk u)."""
         m, f, u = C.c_int(), C.c_int(), C.c_int()
         _check(lib().ppsd_get_pretest(self._h, C.byref(m), C.byref(f), C.byref(u)), "ppsd_get_pretest")
         return m.value, f.value, u.value

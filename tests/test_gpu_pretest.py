@@ -1,8 +1,10 @@
-"""The walk's two pre-test forms (ppsd_set_pretest): the exact odd-part test
-and the range test K <= kmax*2^(s-1) (or >= p minus that).  Each is only a
-filter in front of the same exact re-scan, so every result must be the same
-under both and equal the oracle's; the range form is forced onto dense units
-where it fires in most chunks, which drives the re-scan path hard."""
+"""The walk's three pre-test forms (ppsd_set_pretest): the exact odd-part
+test, the range test K <= kmax*2^(s-1) (or >= p minus that) and the odd-low
+test (the low 32 bits of the odd part, minimised over a chunk, <= kmax).
+Each is only a filter in front of the same exact re-scan, so every result
+must be the same under all three and equal the oracle's; the range and
+odd-low forms are forced onto dense units where they fire in most chunks,
+which drives the re-scan path hard."""
 import json
 import os
 import random
@@ -17,14 +19,15 @@ from conftest import GOLDEN, read_lines
 pytestmark = pytest.mark.gpu
 
 RUNS = json.load(open(os.path.join(GOLDEN, "runs.json")))
-EXACT, RANGE = 1, 2
+EXACT, RANGE, ODDLO = 1, 2, 3
+FORMS = (EXACT, RANGE, ODDLO)
 
 
 def golden_set(name):
     return {orc.parse_line(l) for l in read_lines(name)}
 
 
-@pytest.mark.parametrize("mode", (EXACT, RANGE))
+@pytest.mark.parametrize("mode", FORMS)
 @pytest.mark.parametrize("tag,name", [("cfg2_slice", "cfg2_slice.txt"), ("cfg3_slice", "cfg3_slice.txt"),
                                       ("many_n", "many_n_filtered.txt")])
 def test_golden_ranges_each_form(mode, tag, name):
@@ -58,8 +61,9 @@ def test_dense_units_each_form(seed):
     want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
     with ppsd.Context(ppsd.manual_params(**kw)) as ctx:
         ctx.set_tuning(r=1 + seed % 2, u=(4, 8, 16)[seed % 3])
-        for mode in (EXACT, RANGE):
+        for mode in FORMS:
             ctx.set_pretest(mode)
+            assert ctx.pretest()[1] == mode
             raw, cnt, _ = ctx.run_range(pmin, pmax, raw=True)
             got, _, _ = ctx.run_range(pmin, pmax)
             assert cnt == len(P)
@@ -70,20 +74,87 @@ def test_dense_units_each_form(seed):
 def test_automatic_choice():
     """Automatic: the range form, on a chunk of 4 windows, where
     kmax*2^(s-1)/p is tiny (2^61, both signs, kmax 1e5: 0.05 false alarms
-    per chunk of a wave); the exact form where it is not (2^50, kmax 9999,
-    s = 32: about 10 per chunk); a forced form keeps the tuned chunk."""
-    for tag, want in (("cfg3_slice", (0, RANGE, 4)), ("cfg2_slice", (0, EXACT, 8))):
-        with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS[tag]["params"]))) as ctx:
+    per chunk of a wave); the odd-low form where kmax/2^32 is (2^50 and
+    4.2e13 with kmax 9999: 1.2e-3 false alarms per chunk); the exact form
+    where neither is (2^46, kmax 1e6); a forced form keeps the tuned chunk."""
+    big_k = dict(pmin=2**46, pmax=2**46 + 10**6, kmin=3, kmax=999999, nmin=100000, nmax=2000000)
+    for tag, kw, want in (("cfg3_slice", RUNS["cfg3_slice"]["params"], (0, RANGE, 4)),
+                          ("cfg2_slice", RUNS["cfg2_slice"]["params"], (0, ODDLO, 8)),
+                          ("many_n", RUNS["many_n"]["params"], (0, ODDLO, 8)),
+                          ("2^46 kmax 1e6", big_k, (0, EXACT, 8))):
+        with ppsd.Context(ppsd.derive(ppsd.SearchParams(**kw))) as ctx:
             ctx.set_tuning(r=1, u=8)
             assert ctx.pretest() == want, tag
-            other = EXACT if want[1] == RANGE else RANGE
-            ctx.set_pretest(other)
-            assert ctx.pretest() == (other, other, 8)
+            for other in FORMS:
+                ctx.set_pretest(other)
+                assert ctx.pretest() == (other, other, 8)
     with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS["cfg2_slice"]["params"]))) as ctx:
         with pytest.raises(ppsd.PPSDError):
-            ctx.set_pretest(3)
+            ctx.set_pretest(4)
         with pytest.raises(ppsd.PPSDError):
             ctx.set_pretest(-1)
+
+
+def _manual(kw):
+    """Parameters with the largest window that keeps one candidate per
+    window (kmax*2^(s-1) < pmin), outside the reference's rules."""
+    s = 32
+    while (kw["kmax"] << (s - 1)) >= kw["pmin"]:
+        s -= 1
+    params = ppsd.manual_params(nstep=s, **kw)
+    pr = orc.OrcParams(kstep=2, koffset=1, nstep_ref=s, nstep=s, **kw)
+    return params, pr
+
+
+def test_oddlo_needs_kmax_below_2p32():
+    """kmax >= 2^32: a forced odd-low form falls back to the exact form, and
+    the results are the oracle's."""
+    pmin = 2**45 + 1
+    kw = dict(pmin=pmin, pmax=pmin + 4000, kmin=2**33 - 2**26 + 1, kmax=2**33 + 1, nmin=64, nmax=30000, sign=0)
+    params, pr = _manual(kw)
+    P = orc.primes(kw["pmin"], kw["pmax"])
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+    assert want_raw
+    with ppsd.Context(params) as ctx:
+        ctx.set_pretest(ODDLO)
+        assert ctx.pretest()[:2] == (ODDLO, EXACT)
+        raw, cnt, _ = ctx.run_range(kw["pmin"], kw["pmax"], raw=True)
+    assert cnt == len(P)
+    assert ppsd.factor_set(raw) == want_raw
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oddlo_extremes(seed):
+    """The odd-low form where its compare is tightest and loosest: kmax of 32
+    bits (odd parts compared in the full low word), and p just above 2^32
+    with kmax of 10 bits, where about half of all hits are states below 2^32
+    and many states have a zero low word; every sign; raw divisors as the
+    oracle's."""
+    rng = random.Random(7700 + seed)
+    if seed % 2 == 0:
+        kmax = rng.randrange(2**31, 2**32) | 1
+        pmin = rng.randrange(2**40, 2**42)
+        kmin = kmax - rng.randrange(2**28, 2**31)
+    else:
+        kmax = rng.randrange(2**9, 2**10) | 1
+        pmin = rng.randrange(2**32 + 1, 2**33)
+        kmin = rng.randrange(1, kmax)
+    sign = (1, -1, 0)[seed % 3]
+    nmin = rng.randrange(64, 2000)
+    span = rng.randrange(4000, 20000) if seed % 2 == 0 else rng.randrange(10**6, 2 * 10**6)
+    kw = dict(pmin=pmin, pmax=pmin + span, kmin=kmin, kmax=kmax, nmin=nmin,
+              nmax=nmin + rng.randrange(20000, 60000), sign=sign)
+    params, pr = _manual(kw)
+    P = orc.primes(kw["pmin"], kw["pmax"])
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+    assert want_raw
+    with ppsd.Context(params) as ctx:
+        ctx.set_tuning(r=1 + seed % 2, u=(4, 16)[seed % 2])
+        ctx.set_pretest(ODDLO)
+        assert ctx.pretest()[1] == ODDLO
+        raw, cnt, _ = ctx.run_range(kw["pmin"], kw["pmax"], raw=True)
+    assert cnt == len(P)
+    assert ppsd.factor_set(raw) == want_raw
 
 
 def test_batch_mix_each_form():
@@ -96,13 +167,13 @@ def test_batch_mix_each_form():
         p.pmax = min(p.pmax, p.pmin + 2000000)
     per_mode = {}
     with ppsd.Context(params[0]) as ctx:
-        for mode in (EXACT, RANGE):
+        for mode in FORMS:
             ctx.set_pretest(mode)
             out, unit, cnt, sums = ctx.batch_run(params, raw=True)
             hits = {(int(u), (int(f["p"]), int(f["k"]), int(f["n"]), int(f["c"]))) for f, u in zip(out, unit)}
             assert len(hits) == len(out)
             per_mode[mode] = (hits, [int(x) for x in cnt], [int(x) for x in sums])
-    assert per_mode[EXACT] == per_mode[RANGE]
+    assert per_mode[EXACT] == per_mode[RANGE] == per_mode[ODDLO]
     for i in (0, 5, 11, 23):
         with ppsd.Context(params[i]) as ctx:
             ctx.set_pretest(RANGE)
