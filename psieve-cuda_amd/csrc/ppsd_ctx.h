@@ -73,8 +73,10 @@ struct ppsd_ctx {
   hipStream_t stream = nullptr;
   ppsd_params prm{};
   uint32_t batch = 0;
-  int tune_r = 1, tune_u = 8, tune_wg = 256;
-  int pre_mode = 0;  // ppsd_set_pretest: 0 automatic, 1 exact, 2 range
+  // the shapes the autotuner found fastest on MI355X (256 CUs); callers that
+  // cannot read its cache (the C hosts) get them by default
+  int tune_r = 1, tune_u = 16, tune_wg = 64;
+  int pre_mode = 0;  // ppsd_set_pretest: 0 automatic, 1 exact, 2 range, 3 odd-low
 
   // device buffers
   uint64_t *d_P = nullptr;
@@ -138,7 +140,7 @@ struct ppsd_ctx {
   hipGraph_t b_graph[2] = {};           // per pool: the round graph
   hipGraphExec_t b_exec[2] = {};
   int b_graph_key[2] = {-1, -1};
-  int b_r = 1, b_u = 8, b_waves = 8;
+  int b_r = 2, b_u = 8, b_waves = 8;
   uint32_t b_seg_min = 2048, b_seg_max = 65536;
   BHit *b_out = nullptr;  // the batch's results so far, on the device, unit-indexed
   uint64_t b_out_cap = 0, b_out_n = 0;
