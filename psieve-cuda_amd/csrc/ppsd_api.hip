@@ -469,14 +469,14 @@ ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, b
   const double signs = p.sign == 0 ? 2.0 : 1.0;
   const double f_range = signs * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
   const double f_odd = signs * ldexp((double)p.kmax + 1.0, -32);
-  const double base_range = p.sign == 0 ? 0.56 : 0.75;
-  const double base_odd = p.sign == 0 ? 0.89 : 0.93;
+  const double base_range = p.sign == 0 ? 0.52 : 0.75;
+  const double base_odd = p.sign == 0 ? 0.89 : 0.92;
   WalkForm best{PRE_EXACT, u};
   double best_cost = 1.0;
   for (int cu = u, halvings = 0; cu >= 4 && (free_u || cu == u); cu /= 2, ++halvings) {
     // the range form must win by a margin of 0.03, the odd-low form by 0.01
-    const double range = base_range + 1.95 * f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
-    const double odd = base_odd + 1.95 * f_odd * 64.0 * r * cu + 0.02 * halvings + 0.01;
+    const double range = base_range + f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
+    const double odd = base_odd + f_odd * 64.0 * r * cu + 0.02 * halvings + 0.01;
     if (range < best_cost) {
       best_cost = range;
       best = {PRE_RANGE, cu};

@@ -84,10 +84,23 @@ __device__ __forceinline__ uint64_t add_u32_u64(uint32_t a, uint64_t b) {
   return r;
 }
 
+//
+// At s = 32 the middle mad's 64-bit addend is hi32(K) zero-extended, an
+// even-aligned register pair whose high register must hold 0.  `zero` is 0
+// from opaque_zero(): the compiler cannot re-create it with a move, so it
+// keeps it in one register and each window costs a single v_mov_b32 (of
+// hi32(K) into the register below it).  With a literal 0 the register
+// allocator sometimes rewrote the zero every window as well.
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 template <bool S32>
-__device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s) {
+__device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s, uint32_t zero) {
   const uint32_t m = lo32(K) * ps_sh;
-  const uint64_t ks = S32 ? (uint64_t)hi32(K) : (K >> s);
+  const uint64_t ks = S32 ? ((uint64_t)zero << 32 | hi32(K)) : (K >> s);
   const uint64_t u = (uint64_t)m * hi32(p) + ks;
   const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
   return add_u32_u64(hc, u);

@@ -166,11 +166,12 @@ double hits_per_prime(const ppsd_params &p, uint64_t lo);
 // PRE_RANGE or PRE_ODDLO) and the chunk length.  mode 1, 2 and 3 force a
 // form (3 falls back to PRE_EXACT when kmax >= 2^32); mode 0 prices them
 // with costs measured on MI355X, relative to the exact form's time per
-// window: the range form costs 0.56 (both signs) or 0.75 (one sign), the
-// odd-low form 0.89 or 0.93, each plus 1.95 per expected false alarm of a
-// wave's chunk (64*r*u times the alarm rate per state: bound/p for the range
-// form, kmax/2^32 for the other; twice that for both signs) and 0.02 for
-// each halving of the chunk below u.  Where chunks are shared by many units
+// window: the range form costs 0.52 (both signs) or 0.75 (one sign), the
+// odd-low form 0.89 or 0.92, each plus 1.0 per expected false alarm of a
+// wave's chunk (a re-walk of the chunk with the exact pre-test; 64*r*u times
+// the alarm rate per state: bound/p for the range form, kmax/2^32 for the
+// other; twice that for both signs) and 0.02 for each halving of the chunk
+// below u.  Where chunks are shared by many units
 // (batched pieces) the chunk stays u (free_u false).
 struct WalkForm {
   int pre;

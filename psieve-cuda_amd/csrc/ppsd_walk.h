@@ -103,34 +103,44 @@ struct LanePrimes {
   uint64_t pmb[R];  // p - bound (range pre-test, both signs)
   uint32_t ps[R];
   bool valid[R];
+  uint32_t zero;    // opaque_zero(), for step()
 };
 template <int R>
 struct LaneStates {
   uint64_t K[R];
 };
 
-// Re-walk `cnt` windows from state K (window start n) with exact tests and
-// emission.  Reached only when a pre-test fired somewhere in the chunk.
+// Re-walk `cnt` windows from state K (window start n).  Reached only when a
+// pre-test fired somewhere in the chunk: a false alarm of a cheap form, or a
+// hit.  Each window first gets the exact pre-test (three instructions per
+// state), and only a window that passes it gets the full test and the
+// emission, so a false alarm costs little more than walking the chunk again.
 template <int SIGN, int R, bool S32, class Sink>
 __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
                                        const WindowRule &a, const Sink &sink) {
   for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
+    bool any = false;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint64_t Kc = S.K[r];
-      uint64_t k = 0;
-      uint32_t nn = 0;
-      if (SIGN == 0) {
-        bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
-        emit(h, L.p[r], k, nn, -1, sink);
-        h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
-        emit(h, L.p[r], k, nn, +1, sink);
-      } else {
-        const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
-        emit(h, L.p[r], k, nn, SIGN, sink);
+    for (int r = 0; r < R; ++r) any |= pre<SIGN, PRE_EXACT>(S.K[r], L.p[r], L.pmb[r], a);
+    if (__ballot(any)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t Kc = S.K[r];
+        uint64_t k = 0;
+        uint32_t nn = 0;
+        if (SIGN == 0) {
+          bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
+          emit(h, L.p[r], k, nn, -1, sink);
+          h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
+          emit(h, L.p[r], k, nn, +1, sink);
+        } else {
+          const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
+          emit(h, L.p[r], k, nn, SIGN, sink);
+        }
       }
-      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
   }
 }
 
@@ -150,7 +160,7 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
       } else {
         any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
       }
-      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s);
+      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
     }
   }
   if (PRE == PRE_ODDLO) any = lowest <= (uint32_t)a.kmax;
@@ -181,6 +191,7 @@ __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<
 template <int SIGN, int R>
 __device__ __forceinline__ void start_states(LanePrimes<R> &L, LaneStates<R> &S, uint32_t e, uint32_t s,
                                              uint64_t bound) {
+  L.zero = opaque_zero();
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     L.pmb[r] = L.p[r] - bound;
