@@ -43,8 +43,13 @@ static uint32_t cand_rows = 0;
 static pthread_mutex_t factors_mutex = PTHREAD_MUTEX_INITIALIZER;
 
 #define MAX_WORKERS 32
-static ppsd_ctx *ctxs[MAX_WORKERS];
-static int pending[MAX_WORKERS]; /* a submission is on the device */
+/* Two contexts per worker, used in turn: a batch is checked, staged and
+ * launched on one while the other's batch is still walking, so the device
+ * does not wait for the host's share of a submission or of a collection.
+ * ctxs[th][1] is NULL when a second context could not be opened. */
+static ppsd_ctx *ctxs[MAX_WORKERS][2];
+static int pending[MAX_WORKERS][2]; /* a submission is on the device */
+static int next_ctx[MAX_WORKERS];   /* the context of the next submission */
 static ppsd_factor *fbuf[MAX_WORKERS];
 static uint64_t fcap[MAX_WORKERS];
 
@@ -65,20 +70,20 @@ static void device_fail(int th, const char *where, int rc) {
   bexit(rc == PPSD_ERR_INSUFFICIENT_RESOURCE ? ERR_INSUFFICIENT_RESOURCE : EXIT_FAILURE);
 }
 
-/* Wait for thread th's submission and report its factors. */
-static int drain(int th, uint64_t *count, uint64_t *sum) {
+/* Wait for the submission on thread th's context j and report its factors. */
+static int drain(int th, int j, uint64_t *count, uint64_t *sum) {
   uint64_t n = 0, cnt = 0, s = 0;
   int rc;
-  if (!pending[th]) return 0;
-  pending[th] = 0;
-  rc = ppsd_collect(ctxs[th], fbuf[th], fcap[th], &n, &cnt, &s);
+  if (!pending[th][j]) return 0;
+  pending[th][j] = 0;
+  rc = ppsd_collect(ctxs[th][j], fbuf[th], fcap[th], &n, &cnt, &s);
   if (rc) return rc;
   if (n > fcap[th]) {
     free(fbuf[th]);
     fcap[th] = n + n / 2;
     fbuf[th] = malloc(fcap[th] * sizeof(ppsd_factor));
     if (!fbuf[th]) return PPSD_ERR_INSUFFICIENT_RESOURCE;
-    rc = ppsd_collect(ctxs[th], fbuf[th], fcap[th], &n, &cnt, &s);
+    rc = ppsd_collect(ctxs[th][j], fbuf[th], fcap[th], &n, &cnt, &s);
     if (rc) return rc;
   }
   for (uint64_t i = 0; i < n; i++) report_factor(&fbuf[th][i]);
@@ -233,36 +238,60 @@ unsigned int app_thread_init(int th) {
   int rc;
   if (th < 0 || th >= MAX_WORKERS) return 0;
   if (user_cthread_count) batch = user_cthread_count * 256u;
-  rc = ppsd_open(dev, &params, batch, &ctxs[th]);
+  rc = ppsd_open(dev, &params, batch, &ctxs[th][0]);
   if (rc) {
     fprintf(stderr, "%sGPU %d not usable: %s\n", bmprefix(), dev, ppsd_last_error());
     if (rc == PPSD_ERR_INVALID_PARAM) bexit(ERR_INVALID_PARAM);
     return 0;
   }
-  if (cand_bitmap) {
-    rc = ppsd_set_candidates(ctxs[th], cand_bitmap, nmin, cand_rows, kmin / kstep, cand_row_bytes);
+  if (ppsd_open(dev, &params, ppsd_batch_size(ctxs[th][0]), &ctxs[th][1])) ctxs[th][1] = NULL;
+  for (int j = 0; j < 2 && cand_bitmap; j++) {
+    if (!ctxs[th][j]) continue;
+    rc = ppsd_set_candidates(ctxs[th][j], cand_bitmap, nmin, cand_rows, kmin / kstep, cand_row_bytes);
     if (rc) {
       fprintf(stderr, "%sGPU %d: candidate list: %s\n", bmprefix(), dev, ppsd_last_error());
-      ppsd_close(ctxs[th]);
-      ctxs[th] = NULL;
+      app_thread_fini(th);
       return 0;
     }
   }
+  next_ctx[th] = 0;
   fcap[th] = 1024;
   fbuf[th] = malloc(fcap[th] * sizeof(ppsd_factor));
-  fprintf(stderr, "%sThread %d: GPU %d, %u primes per batch\n", bmprefix(), th, dev, ppsd_batch_size(ctxs[th]));
-  return ppsd_batch_size(ctxs[th]);
+  fprintf(stderr, "%sThread %d: GPU %d, %u primes per batch\n", bmprefix(), th, dev, ppsd_batch_size(ctxs[th][0]));
+  return ppsd_batch_size(ctxs[th][0]);
 }
 
-/* A batch of host primes (pps/app.c:1224-1250): finish the previous batch,
- * start this one, return without waiting.  The device copies P, so lastP
- * is not needed to remember the batch in flight. */
+/* The context for thread th's next submission, once its previous batch
+ * (two submissions back) is collected. */
+static int take_ctx(int th, uint64_t *count, uint64_t *sum, int *rc) {
+  const int j = ctxs[th][1] ? next_ctx[th] : 0;
+  *rc = drain(th, j, count, sum);
+  next_ctx[th] = j ^ 1;
+  return j;
+}
+
+/* Collect both of thread th's contexts, the older submission first. */
+static int drain_all(int th, uint64_t *count, uint64_t *sum) {
+  for (int k = 0; k < 2; k++) {
+    const int j = (next_ctx[th] + k) & 1;
+    if (!ctxs[th][j]) continue;
+    const int rc = drain(th, j, count, sum);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* A batch of host primes (pps/app.c:1224-1250): collect the batch that was
+ * submitted two calls ago, start this one, return without waiting; the
+ * previous call's batch keeps walking meanwhile.  The device copies P, so
+ * lastP is not needed to remember the batches in flight. */
 void app_thread_fun(int th, const uint64_t *P, uint64_t *lastP, const unsigned int cthread_count) {
-  int rc = drain(th, NULL, NULL);
+  int rc;
+  const int j = take_ctx(th, NULL, NULL, &rc);
   if (rc) device_fail(th, "collect", rc);
-  rc = ppsd_submit_primes(ctxs[th], P, cthread_count);
+  rc = ppsd_submit_primes(ctxs[th][j], P, cthread_count);
   if (rc) device_fail(th, "submit_primes", rc);
-  pending[th] = 1;
+  pending[th][j] = 1;
 }
 
 /* Flush (pps/app.c:1257-1286): the last len primes, then drain everything,
@@ -271,10 +300,12 @@ void app_thread_fun(int th, const uint64_t *P, uint64_t *lastP, const unsigned i
 void app_thread_fun1(int th, uint64_t *P, uint64_t *lastP, const unsigned int cthread_count, unsigned int len) {
   int rc;
   if (len > 0) app_thread_fun(th, P, lastP, len);
-  rc = drain(th, NULL, NULL);
+  rc = drain_all(th, NULL, NULL);
   if (rc) device_fail(th, "collect", rc);
 }
 
+/* All primes of [p0, p1) sieved on the device, in chunks that alternate
+ * between the two contexts; returns with nothing in flight. */
 int app_thread_range(int th, uint64_t p0, uint64_t p1, uint64_t *count, uint64_t *sum) {
   uint64_t p = p0;
   int rc;
@@ -282,19 +313,22 @@ int app_thread_range(int th, uint64_t p0, uint64_t p1, uint64_t *count, uint64_t
   *sum = 0;
   while (p < p1) {
     uint64_t next = p1;
-    rc = ppsd_submit_range(ctxs[th], p, p1, &next);
+    const int j = take_ctx(th, count, sum, &rc);
     if (rc) return rc;
-    pending[th] = 1;
-    rc = drain(th, count, sum);
+    rc = ppsd_submit_range(ctxs[th][j], p, p1, &next);
     if (rc) return rc;
+    pending[th][j] = 1;
     p = next;
   }
-  return 0;
+  return drain_all(th, count, sum);
 }
 
 void app_thread_fini(int th) {
-  ppsd_close(ctxs[th]);
-  ctxs[th] = NULL;
+  for (int j = 0; j < 2; j++) {
+    if (ctxs[th][j]) ppsd_close(ctxs[th][j]);
+    ctxs[th][j] = NULL;
+    pending[th][j] = 0;
+  }
   free(fbuf[th]);
   fbuf[th] = NULL;
 }
