@@ -97,13 +97,43 @@ __device__ __forceinline__ uint32_t opaque_zero() {
   return z;
 }
 
+// a*b + c as one v_mad_u64_u32.  Written as asm: with a second, fused walk
+// loop beside it (step32_fused) the compiler widened the plain expression's
+// product to 64 x 32 bits, two mads per product.
+__device__ __forceinline__ uint64_t mad_u64_u32(uint32_t a, uint32_t b, uint64_t c) {
+  uint64_t r, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 template <bool S32>
 __device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s, uint32_t zero) {
   const uint32_t m = lo32(K) * ps_sh;
   const uint64_t ks = S32 ? ((uint64_t)zero << 32 | hi32(K)) : (K >> s);
-  const uint64_t u = (uint64_t)m * hi32(p) + ks;
-  const uint32_t hc = hi32((uint64_t)m * lo32(p) + 0xffffffffull);
+  const uint64_t u = mad_u64_u32(m, hi32(p), ks);
+  const uint32_t hc = hi32(mad_u64_u32(m, lo32(p), 0xffffffffull));
   return add_u32_u64(hc, u);
+}
+
+// The window step at s = 32 in three multiplies, for primes whose words
+// satisfy lo32(p) + hi32(p) < 2^32.  The ceiling term and hi32(K) go into one
+// addend: the 64-bit value S = m*p_lo + (hi32(K)*2^32 + 2^32 - 1) has the
+// high word hi32(K) + ceil(m*p_lo / 2^32), so K' = m*p_hi + hi32(S).  That
+// needs S < 2^64: the ceiling is at most p_lo (m < 2^32) and hi32(K) at most
+// p_hi (K < p), so the high word stays below 2^32 when p_lo + p_hi < 2^32.
+// The step is then one v_mul_lo_u32 and two v_mad_u64_u32; the two moves
+// that place the addends' words (2^32 - 1 below hi32(K), hi32(S) below the
+// zero register) are v_mov_b32.  Primes that fail the bound lie at the top
+// of each 2^32 block (p_lo > 2^32 - 1 - p_hi), so in an ascending batch they
+// form short runs and whole waves take one step or the other.
+__device__ __forceinline__ bool fused_step_ok(uint64_t p) {
+  return (uint64_t)lo32(p) + hi32(p) <= 0xffffffffull;
+}
+__device__ __forceinline__ uint64_t step32_fused(uint64_t K, uint64_t p, uint32_t ps, uint32_t zero) {
+  const uint32_t m = lo32(K) * ps;
+  const uint64_t a = (uint64_t)hi32(K) << 32 | 0xffffffffull;
+  const uint32_t x = hi32(mad_u64_u32(m, lo32(p), a));
+  return mad_u64_u32(m, hi32(p), (uint64_t)zero << 32 | x);
 }
 
 // The multiplier of step() for prime p and window s.

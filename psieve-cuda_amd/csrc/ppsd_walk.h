@@ -145,8 +145,9 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
 }
 
 // Pre-test of W windows of R states each, stepping them: true when one of
-// them may hold a hit.
-template <int SIGN, int R, int W, bool S32, int PRE>
+// them may hold a hit.  FUSED (s = 32 only): the three-multiply step, for a
+// wave whose primes all pass fused_step_ok.
+template <int SIGN, int R, int W, bool S32, int PRE, bool FUSED>
 __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a) {
   bool any = false;
   uint32_t lowest = ~0u;
@@ -160,30 +161,48 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
       } else {
         any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
       }
-      S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
+      S.K[r] = FUSED ? step32_fused(S.K[r], L.p[r], L.ps[r], L.zero)
+                     : step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
     }
   }
   if (PRE == PRE_ODDLO) any = lowest <= (uint32_t)a.kmax;
   return any;
 }
 
-// Walk nwin windows from window start n, states S.  Steps run in chunks of U
-// and a chunk only leaves the straight-line path when its pre-test fired
-// (then it is walked again with exact tests).
-template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
-__device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
-                                             const WindowRule &a, const Sink &sink) {
+template <int SIGN, int R, int U, bool S32, int PRE, bool FUSED, class Sink>
+__device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
+                                            const WindowRule &a, const Sink &sink) {
   const uint32_t s = a.s;
   uint32_t w = 0;
   for (; w + U <= nwin; w += U) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, U, S32, PRE>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
+    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
     n += U * s;
   }
   for (; w < nwin; ++w, n += s) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, 1, S32, PRE>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, 1, a, sink);
+    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, 1, a, sink);
   }
+}
+
+// Walk nwin windows from window start n, states S.  Steps run in chunks of U
+// and a chunk only leaves the straight-line path when its pre-test fired
+// (then it is walked again with exact tests).  At s = 32 a wave whose primes
+// all pass fused_step_ok walks with the three-multiply step (a wave-uniform
+// choice, so no lane diverges).
+template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
+__device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
+                                             const WindowRule &a, const Sink &sink) {
+  if (S32) {
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < R; ++r) ok = ok && fused_step_ok(L.p[r]);
+    if (__all(ok)) {
+      walk_chunks<SIGN, R, U, S32, PRE, true>(L, S, n, nwin, a, sink);
+      return;
+    }
+  }
+  walk_chunks<SIGN, R, U, S32, PRE, false>(L, S, n, nwin, a, sink);
 }
 
 // Start states: K = -c * 2^-e mod p (both signs: 2^-e, whose negation p - K

@@ -47,12 +47,19 @@ def test_state_vectors_against_pow():
 def test_every_state_stays_canonical():
     """Each of the first 160 states of a walk equals 2^-n mod p exactly (not
     merely mod p): the walk's pre-test relies on canonical states.  Primes
-    whose low word is extreme stress the ceiling term of the step."""
+    whose low word is extreme stress the ceiling term of the step.  At s = 32
+    the device takes the three-multiply step for primes with
+    lo32(p) + hi32(p) < 2^32 and the four-multiply one for the rest: the
+    primes on either side of that bound cover both."""
     rng = random.Random(6)
     P = [sympy.nextprime(rng.randrange(2**32, 2**62)) for _ in range(100)]
     for hi in (1, 2**20, 2**29, 2**30 - 1):
         P += [sympy.prevprime(hi * 2**32 + 2**32 - 1), sympy.nextprime(hi * 2**32)]
+        edge = hi * 2**32 + 2**32 - 1 - hi  # lo32 + hi32 == 2^32 - 1
+        P += [sympy.prevprime(edge + 1), sympy.nextprime(edge)]
     P += [sympy.prevprime(2**62)]
+    fused = [(p & 0xFFFFFFFF) + (p >> 32) < 2**32 for p in P]
+    assert 20 < sum(fused) < len(P) - 5
     P = np.array(P, dtype=np.uint64)
     for s in (32, 31, 7, 1):
         for nwin in range(160):
