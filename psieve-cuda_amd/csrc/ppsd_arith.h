@@ -115,25 +115,30 @@ __device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh,
   return add_u32_u64(hc, u);
 }
 
-// The window step at s = 32 in three multiplies, for primes whose words
-// satisfy lo32(p) + hi32(p) < 2^32.  The ceiling term and hi32(K) go into one
-// addend: the 64-bit value S = m*p_lo + (hi32(K)*2^32 + 2^32 - 1) has the
-// high word hi32(K) + ceil(m*p_lo / 2^32), so K' = m*p_hi + hi32(S).  That
-// needs S < 2^64: the ceiling is at most p_lo (m < 2^32) and hi32(K) at most
-// p_hi (K < p), so the high word stays below 2^32 when p_lo + p_hi < 2^32.
-// The step is then one v_mul_lo_u32 and two v_mad_u64_u32; the two moves
-// that place the addends' words (2^32 - 1 below hi32(K), hi32(S) below the
-// zero register) are v_mov_b32.  Primes that fail the bound lie at the top
-// of each 2^32 block (p_lo > 2^32 - 1 - p_hi), so in an ascending batch they
-// form short runs and whole waves take one step or the other.
-__device__ __forceinline__ bool fused_step_ok(uint64_t p) {
-  return (uint64_t)lo32(p) + hi32(p) <= 0xffffffffull;
+// The window step in three multiplies, for primes with
+// ((p-1) >> s) + lo32(p) < 2^32 (at s = 32: lo32(p) + hi32(p) < 2^32).
+// The ceiling term and X = K >> s go into one addend: the 64-bit value
+// S = m*p_lo + (X*2^32 + 2^32 - 1) has the high word X + ceil(m*p_lo / 2^32),
+// so K' = m*p_hi + hi32(S).  That needs X < 2^32 and S < 2^64: the ceiling
+// is at most p_lo (m < 2^32) and X at most (p-1) >> s (K < p), so the bound
+// above gives both.  X is hi32(K) at s = 32 and v_alignbit_b32(hi32(K),
+// lo32(K), s) -- the low word of K >> s, all of it here -- below.  The step
+// is then one v_mul_lo_u32 and two v_mad_u64_u32 (plus v_alignbit_b32 at
+// s < 32, in place of the 64-bit shift of step()); the moves that place the
+// addends' words (2^32 - 1 below X, hi32(S) below the zero register) are
+// v_mov_b32.  Primes that fail the bound have lo32(p) within (p-1) >> s of
+// 2^32, i.e. they lie at the top of each 2^32 block, so in an ascending
+// batch they form runs and whole waves take one step or the other.
+__device__ __forceinline__ bool fused_step_ok(uint64_t p, uint32_t s) {
+  return ((p - 1) >> s) + lo32(p) <= 0xffffffffull;
 }
-__device__ __forceinline__ uint64_t step32_fused(uint64_t K, uint64_t p, uint32_t ps, uint32_t zero) {
-  const uint32_t m = lo32(K) * ps;
-  const uint64_t a = (uint64_t)hi32(K) << 32 | 0xffffffffull;
-  const uint32_t x = hi32(mad_u64_u32(m, lo32(p), a));
-  return mad_u64_u32(m, hi32(p), (uint64_t)zero << 32 | x);
+template <bool S32>
+__device__ __forceinline__ uint64_t step_fused(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s, uint32_t zero) {
+  const uint32_t m = lo32(K) * ps_sh;
+  const uint32_t x = S32 ? hi32(K) : __builtin_amdgcn_alignbit(hi32(K), lo32(K), s);
+  const uint64_t a = (uint64_t)x << 32 | 0xffffffffull;
+  const uint32_t hs = hi32(mad_u64_u32(m, lo32(p), a));
+  return mad_u64_u32(m, hi32(p), (uint64_t)zero << 32 | hs);
 }
 
 // The multiplier of step() for prime p and window s.

@@ -383,8 +383,10 @@ __global__ __launch_bounds__(256) void state_kernel(const uint64_t *P, uint32_t 
   const uint32_t pinv_sh = step_multiplier(pinv, s);
   uint64_t K = inv2pow(p, pinv, e);
   const uint32_t zero = opaque_zero();
-  if (s == 32 && fused_step_ok(p))
-    for (uint32_t w = 0; w < nwin; ++w) K = step32_fused(K, p, pinv_sh, zero);
+  if (s == 32 && fused_step_ok(p, s))
+    for (uint32_t w = 0; w < nwin; ++w) K = step_fused<true>(K, p, pinv_sh, s, zero);
+  else if (fused_step_ok(p, s))
+    for (uint32_t w = 0; w < nwin; ++w) K = step_fused<false>(K, p, pinv_sh, s, zero);
   else if (s == 32)
     for (uint32_t w = 0; w < nwin; ++w) K = step<true>(K, p, pinv_sh, s, zero);
   else

@@ -145,8 +145,8 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
 }
 
 // Pre-test of W windows of R states each, stepping them: true when one of
-// them may hold a hit.  FUSED (s = 32 only): the three-multiply step, for a
-// wave whose primes all pass fused_step_ok.
+// them may hold a hit.  FUSED: the three-multiply step, for a wave whose
+// primes all pass fused_step_ok.
 template <int SIGN, int R, int W, bool S32, int PRE, bool FUSED>
 __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a) {
   bool any = false;
@@ -161,7 +161,7 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
       } else {
         any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
       }
-      S.K[r] = FUSED ? step32_fused(S.K[r], L.p[r], L.ps[r], L.zero)
+      S.K[r] = FUSED ? step_fused<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero)
                      : step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
     }
   }
@@ -187,22 +187,19 @@ __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R
 
 // Walk nwin windows from window start n, states S.  Steps run in chunks of U
 // and a chunk only leaves the straight-line path when its pre-test fired
-// (then it is walked again with exact tests).  At s = 32 a wave whose primes
-// all pass fused_step_ok walks with the three-multiply step (a wave-uniform
-// choice, so no lane diverges).
+// (then it is walked again with exact tests).  A wave whose primes all pass
+// fused_step_ok walks with the three-multiply step (a wave-uniform choice,
+// so no lane diverges).
 template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
 __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                              const WindowRule &a, const Sink &sink) {
-  if (S32) {
-    bool ok = true;
+  bool ok = true;
 #pragma unroll
-    for (int r = 0; r < R; ++r) ok = ok && fused_step_ok(L.p[r]);
-    if (__all(ok)) {
-      walk_chunks<SIGN, R, U, S32, PRE, true>(L, S, n, nwin, a, sink);
-      return;
-    }
-  }
-  walk_chunks<SIGN, R, U, S32, PRE, false>(L, S, n, nwin, a, sink);
+  for (int r = 0; r < R; ++r) ok = ok && fused_step_ok(L.p[r], a.s);
+  if (__all(ok))
+    walk_chunks<SIGN, R, U, S32, PRE, true>(L, S, n, nwin, a, sink);
+  else
+    walk_chunks<SIGN, R, U, S32, PRE, false>(L, S, n, nwin, a, sink);
 }
 
 // Start states: K = -c * 2^-e mod p (both signs: 2^-e, whose negation p - K

@@ -68,6 +68,39 @@ def test_every_state_stays_canonical():
             assert got.tolist() == want, (s, nwin)
 
 
+def _fused_edge(h: int, s: int) -> int:
+    """Largest p = h*2^32 + lo with ((p-1) >> s) + lo <= 2^32 - 1."""
+    lo, hi = 1, 2**32 - 1
+    ok = lambda x: ((h * 2**32 + x - 1) >> s) + x <= 2**32 - 1  # noqa: E731
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        lo, hi = (mid, hi) if ok(mid) else (lo, mid - 1)
+    return h * 2**32 + lo
+
+
+@pytest.mark.parametrize("s", [31, 24, 13, 7, 1])
+def test_fused_step_bound_each_window(s):
+    """The three-multiply step at s < 32 is exact while
+    ((p-1) >> s) + lo32(p) < 2^32, and the device state kernel takes it for
+    exactly those primes.  Primes on both sides of that bound, for several
+    high words, plus random primes small enough for it, walk their first
+    states to 2^-n mod p exactly."""
+    rng = random.Random(100 + s)
+    P = []
+    for h in sorted({1, min(2 ** (s - 1), 2**30 - 1), min(2**s - 1, 2**30 - 1)}):
+        edge = _fused_edge(h, s)
+        P += [sympy.prevprime(edge + 1), sympy.nextprime(edge)]
+    top = min(2 ** (32 + s), 2**62)
+    P += [sympy.nextprime(rng.randrange(2**32, top)) for _ in range(40)]
+    fused = [((p - 1) >> s) + (p & 0xFFFFFFFF) < 2**32 for p in P]
+    assert any(fused) and not all(fused)
+    P = np.array(P, dtype=np.uint64)
+    for nwin in list(range(0, 40)) + [97, 160]:
+        got = ppsd.debug_state(P, 100, nwin, s)
+        want = [pow(2, -(100 + nwin * s), int(p)) for p in P]
+        assert got.tolist() == want, (s, nwin)
+
+
 def test_many_n_raw_and_filtered():
     """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
     pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
