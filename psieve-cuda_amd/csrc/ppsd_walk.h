@@ -115,7 +115,8 @@ struct LaneStates {
 // hit.  Each window first gets the exact pre-test (three instructions per
 // state), and only a window that passes it gets the full test and the
 // emission, so a false alarm costs little more than walking the chunk again.
-template <int SIGN, int R, bool S32, class Sink>
+// FUSED: the wave's step (step_fused), as in test_and_step.
+template <int SIGN, int R, bool S32, bool FUSED, class Sink>
 __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
                                        const WindowRule &a, const Sink &sink) {
   for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
@@ -140,7 +141,9 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
       }
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) S.K[r] = step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
+    for (int r = 0; r < R; ++r)
+      S.K[r] = FUSED ? step_fused<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero)
+                     : step<S32>(S.K[r], L.p[r], L.ps[r], a.s, L.zero);
   }
 }
 
@@ -176,12 +179,12 @@ __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R
   uint32_t w = 0;
   for (; w + U <= nwin; w += U) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, U, a, sink);
+    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED>(L, snap, n, U, a, sink);
     n += U * s;
   }
   for (; w < nwin; ++w, n += s) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32>(L, snap, n, 1, a, sink);
+    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED>(L, snap, n, 1, a, sink);
   }
 }
 
