@@ -98,7 +98,7 @@ __device__ __forceinline__ uint32_t opaque_zero() {
 }
 
 // a*b + c as one v_mad_u64_u32.  Written as asm: with a second, fused walk
-// loop beside it (step32_fused) the compiler widened the plain expression's
+// loop beside it (step_fused) the compiler widened the plain expression's
 // product to 64 x 32 bits, two mads per product.
 __device__ __forceinline__ uint64_t mad_u64_u32(uint32_t a, uint32_t b, uint64_t c) {
   uint64_t r, carry;
