@@ -101,6 +101,36 @@ def test_fused_step_bound_each_window(s):
         assert got.tolist() == want, (s, nwin)
 
 
+@pytest.mark.parametrize("sign", [1, -1, 0])
+def test_walk_across_fused_step_bound(sign):
+    """A p-range across the run of primes that fail the three-multiply
+    step's bound at the top of a 2^32 block (here s = 25, so the run is the
+    top 2^19 of the block, about 18,000 primes), with primes that pass it on
+    both sides: the plain walk (device and host primes) and the batched walk
+    equal the oracle's divisor and factor sets."""
+    B = 2**44 + 2**32
+    kw = dict(pmin=B - 2**20, pmax=B + 2**19, kmin=3, kmax=1000001, nmin=100000, nmax=200000, sign=sign)
+    params = ppsd.derive(ppsd.SearchParams(**kw))
+    s = params.nstep
+    pr = orc.derive(**kw)
+    P = orc.primes(kw["pmin"], kw["pmax"])
+    slow = sum(1 for p in P.tolist() if ((p - 1) >> s) + (p & 0xFFFFFFFF) >= 2**32)
+    assert s == 25 and 10000 < slow < len(P) - 10000, (s, slow, len(P))
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+    want = orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
+    assert len(want_raw) > 100
+    with ppsd.Context(params) as ctx:
+        raw, cnt, _ = ctx.run_range(kw["pmin"], kw["pmax"], raw=True)
+        got, _, _ = ctx.run_range(kw["pmin"], kw["pmax"])
+        host = ctx.run_primes(P, raw=True)
+        braw, bunit, bcnt, _ = ctx.batch_run([params], raw=True)
+    assert cnt == len(P) and int(bcnt[0]) == len(P)
+    assert ppsd.factor_set(raw) == want_raw
+    assert ppsd.factor_set(host) == want_raw
+    assert ppsd.factor_set(braw) == want_raw
+    assert ppsd.factor_set(got) == want
+
+
 def test_many_n_raw_and_filtered():
     """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
     pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
