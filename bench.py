@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0, help="primes per step (0 = library default 2^21)")
     ap.add_argument("--tune", default="", help="r,u,wg kernel shape")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mix-units", type=int, default=256,
                     help="units of the seeded mixed workload run batched and plain (0 = skip)")
@@ -64,9 +64,11 @@ def cpu_baseline(target_s):
     threads = max(1, min(os.cpu_count() or 1, 16))
     pr = orc.derive(**WORKLOAD)
     P = orc.primes(WORKLOAD["pmin"], WORKLOAD["pmin"] + 40 * 10**6)
+    orc.walk(pr, P[:threads], filter=True, threads=threads)  # first call's one-time costs
+    pilot = 32  # primes per thread: long enough that thread start-up is noise
     t0 = time.perf_counter()
-    orc.walk(pr, P[:threads * 2], filter=True, threads=threads)
-    per = (time.perf_counter() - t0) / 2  # seconds per prime per thread, roughly
+    orc.walk(pr, P[:threads * pilot], filter=True, threads=threads)
+    per = (time.perf_counter() - t0) / pilot  # seconds per prime per thread
     m = int(max(threads * 4, min(len(P), target_s / max(per, 1e-9) * threads)))
     m = (m // threads) * threads
     t0 = time.perf_counter()
