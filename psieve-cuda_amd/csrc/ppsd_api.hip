@@ -469,7 +469,7 @@ ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, b
   const double signs = p.sign == 0 ? 2.0 : 1.0;
   const double f_range = signs * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
   const double f_odd = signs * ldexp((double)p.kmax + 1.0, -32);
-  const double base_range = p.sign == 0 ? 0.52 : 0.75;
+  const double base_range = p.sign == 0 ? 0.47 : 0.69;
   const double base_odd = p.sign == 0 ? 0.89 : 0.92;
   WalkForm best{PRE_EXACT, u};
   double best_cost = 1.0;

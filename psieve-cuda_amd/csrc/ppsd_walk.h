@@ -154,6 +154,9 @@ template <int SIGN, int R, int W, bool S32, int PRE, bool FUSED>
 __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a) {
   bool any = false;
   uint32_t lowest = ~0u;
+  uint32_t highest[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) highest[r] = 0;
 #pragma unroll
   for (int u = 0; u < W; ++u) {
 #pragma unroll
@@ -161,6 +164,13 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
       if (PRE == PRE_ODDLO) {
         lowest = min(lowest, odd_low(S.K[r]));
         if (SIGN == 0) lowest = min(lowest, odd_low(L.p[r] - S.K[r]));
+      } else if (PRE == PRE_RANGE) {
+        // K <= bound needs hi32(K) <= hi32(bound), and K >= p - bound needs
+        // hi32(K) >= hi32(p - bound): a running minimum (and, both signs, a
+        // maximum per prime) of the high word, v_min3_u32 / v_max3_u32 per
+        // two windows, in place of a 64-bit compare per window and sign
+        lowest = min(lowest, hi32(S.K[r]));
+        if (SIGN == 0) highest[r] = max(highest[r], hi32(S.K[r]));
       } else {
         any |= pre<SIGN, PRE>(S.K[r], L.p[r], L.pmb[r], a);
       }
@@ -169,6 +179,12 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
     }
   }
   if (PRE == PRE_ODDLO) any = lowest <= (uint32_t)a.kmax;
+  if (PRE == PRE_RANGE) {
+    any = lowest <= hi32(a.bound);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (SIGN == 0) any |= highest[r] >= hi32(L.pmb[r]);
+  }
   return any;
 }
 
