@@ -131,8 +131,9 @@ int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
 /* Form of the walk's per-window pre-test (no counterpart in the reference,
  * whose kernels pick among test macros at compile time: pps/appcu.cu:779-801).
  * 1: the odd part of the state is <= kmax (ctz, shift, compare); 2: the
- * state is <= kmax*2^(s-1) (or, both signs, >= p minus that) -- one compare
- * per sign, and a chunk re-walk whenever a merely small state fires it; 3:
+ * state's high word is <= that of kmax*2^(s-1) (or, both signs, >= that of
+ * p minus it), a running minimum (maximum) over a chunk of windows -- a
+ * chunk re-walk whenever a merely small state fires it; 3:
  * the low 32 bits of the odd part, minimised over a chunk of windows, are <=
  * kmax -- fires on about kmax/2^32 of all states, for kmax < 2^32 only (else
  * 1 is used); 0 (default): the form, and a chunk
