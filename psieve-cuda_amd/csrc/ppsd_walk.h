@@ -15,11 +15,12 @@ struct WindowRule {
 };
 
 // Pre-test forms.  PRE_EXACT: the odd part of the state (ctz, shift,
-// compare) is <= kmax.  PRE_RANGE: the state itself is <= kmax * 2^(s-1)
-// (both signs: or >= p - that), one compare per sign -- a necessary
-// condition that also fires on states that are merely small, which costs a
-// re-walk of the chunk; the host picks it when that is rare, i.e. when
-// kmax * 2^(s-1) is a tiny fraction of p.  PRE_ODDLO (kmax < 2^32 only): the
+// compare) is <= kmax.  PRE_RANGE: the state's high word is <= that of
+// kmax * 2^(s-1) (both signs: or >= that of p minus it), as a running
+// minimum (maximum) over the chunk -- a necessary condition that also fires
+// on states that are merely small, which costs a re-walk of the chunk; the
+// host picks it when that is rare, i.e. when kmax * 2^(s-1) is a tiny
+// fraction of p.  PRE_ODDLO (kmax < 2^32 only): the
 // low word of the odd part, minimised over the chunk, is <= kmax -- see
 // odd_low.
 enum { PRE_EXACT = 0, PRE_RANGE = 1, PRE_ODDLO = 2 };
