@@ -474,8 +474,10 @@ ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, b
   WalkForm best{PRE_EXACT, u};
   double best_cost = 1.0;
   for (int cu = u, halvings = 0; cu >= 4 && (free_u || cu == u); cu /= 2, ++halvings) {
-    // the range form must win by a margin of 0.03, the odd-low form by 0.01
-    const double range = base_range + f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
+    // the range form must win by a margin of 0.03, the odd-low form by 0.01;
+    // a range-form false alarm re-walks its chunk with the range compare
+    // (about half an exact chunk), an odd-low one with the exact pre-test
+    const double range = base_range + 0.5 * f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
     const double odd = base_odd + f_odd * 64.0 * r * cu + 0.02 * halvings + 0.01;
     if (range < best_cost) {
       best_cost = range;

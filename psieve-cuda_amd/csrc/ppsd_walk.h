@@ -113,17 +113,19 @@ struct LaneStates {
 
 // Re-walk `cnt` windows from state K (window start n).  Reached only when a
 // pre-test fired somewhere in the chunk: a false alarm of a cheap form, or a
-// hit.  Each window first gets the exact pre-test (three instructions per
-// state), and only a window that passes it gets the full test and the
-// emission, so a false alarm costs little more than walking the chunk again.
-// FUSED: the wave's step (step_fused), as in test_and_step.
-template <int SIGN, int R, bool S32, bool FUSED, class Sink>
+// hit.  Each window first gets a per-window pre-test, RPRE: the exact one
+// (three instructions per state), or for the range form its 64-bit compare
+// (one per sign, which is all the alarm needs to be located); only a window
+// that passes it gets the full test and the emission, so a false alarm
+// costs little more than walking the chunk again.  FUSED: the wave's step
+// (step_fused), as in test_and_step.
+template <int SIGN, int R, bool S32, bool FUSED, int RPRE, class Sink>
 __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
                                        const WindowRule &a, const Sink &sink) {
   for (uint32_t u = 0; u < cnt; ++u, n += a.s) {
     bool any = false;
 #pragma unroll
-    for (int r = 0; r < R; ++r) any |= pre<SIGN, PRE_EXACT>(S.K[r], L.p[r], L.pmb[r], a);
+    for (int r = 0; r < R; ++r) any |= pre<SIGN, RPRE>(S.K[r], L.p[r], L.pmb[r], a);
     if (__ballot(any)) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -192,16 +194,17 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
 template <int SIGN, int R, int U, bool S32, int PRE, bool FUSED, class Sink>
 __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                             const WindowRule &a, const Sink &sink) {
+  constexpr int RPRE = PRE == PRE_RANGE ? PRE_RANGE : PRE_EXACT;
   const uint32_t s = a.s;
   uint32_t w = 0;
   for (; w + U <= nwin; w += U) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED>(L, snap, n, U, a, sink);
+    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, U, a, sink);
     n += U * s;
   }
   for (; w < nwin; ++w, n += s) {
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED>(L, snap, n, 1, a, sink);
+    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, 1, a, sink);
   }
 }
 
