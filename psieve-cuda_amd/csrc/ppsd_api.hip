@@ -863,15 +863,45 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
     if (n >= target || seg <= c->b_seg_min) break;
     seg /= 2;
   }
-  tasks.clear();
+  // Largest first.  A piece's tasks come in at most four sizes: full or last
+  // block of primes times full or last segment of windows.  Ordering those
+  // classes (a few per piece) and emitting each class's tasks in turn gives
+  // the size order without sorting the tasks themselves (tens of thousands
+  // a round, which a comparison sort took milliseconds of host time over).
+  struct Class {
+    uint64_t size;
+    uint32_t piece, f0, f1, fstep, w0, w1, wstep;
+  };
+  std::vector<Class> cls;
+  cls.reserve(4 * (size_t)npieces);
+  uint64_t total = 0;
   for (uint32_t i = 0; i < npieces; ++i) {
     const PieceDesc &P = c->h_pieces[i];
-    for (uint32_t f = 0; f < P.nprimes; f += per)
-      for (uint32_t w = 0; w < P.nwin; w += seg)
-        tasks.push_back(Task{i, f, std::min(per, P.nprimes - f), w, std::min(seg, P.nwin - w)});
+    if (!P.nprimes || !P.nwin) continue;
+    const uint32_t nfull = P.nprimes / per, flast = P.nprimes % per;  // full blocks, last block's count
+    const uint32_t wfull = P.nwin / seg, wlast = P.nwin % seg;
+    const uint32_t fend = nfull * per, wend = wfull * seg;
+    // [f0, f1) x [w0, w1) in steps of per and seg: every task of the class
+    // has `count` primes and `nw` windows
+    auto add = [&](uint32_t f0, uint32_t f1, uint32_t count, uint32_t w0, uint32_t w1, uint32_t nw) {
+      if (f0 >= f1 || w0 >= w1) return;
+      cls.push_back(Class{(uint64_t)count * nw, i, f0, f1, per, w0, w1, seg});
+      total += (uint64_t)((f1 - f0 + per - 1) / per) * ((w1 - w0 + seg - 1) / seg);
+    };
+    add(0, fend, per, 0, wend, seg);
+    add(0, fend, per, wend, P.nwin, wlast);
+    add(fend, P.nprimes, flast, 0, wend, seg);
+    add(fend, P.nprimes, flast, wend, P.nwin, wlast);
   }
-  std::stable_sort(tasks.begin(), tasks.end(),
-                   [](const Task &x, const Task &y) { return (uint64_t)x.count * x.nw > (uint64_t)y.count * y.nw; });
+  std::stable_sort(cls.begin(), cls.end(), [](const Class &x, const Class &y) { return x.size > y.size; });
+  tasks.clear();
+  tasks.reserve(total);
+  for (const Class &k : cls)
+    for (uint32_t f = k.f0; f < k.f1; f += k.fstep)
+      for (uint32_t w = k.w0; w < k.w1; w += k.wstep) {
+        const PieceDesc &P = c->h_pieces[k.piece];
+        tasks.push_back(Task{k.piece, f, std::min(per, P.nprimes - f), w, std::min(seg, P.nwin - w)});
+      }
   (void)par;
   return seg;
 }
