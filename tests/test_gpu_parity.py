@@ -131,6 +131,28 @@ def test_walk_across_fused_step_bound(sign):
     assert ppsd.factor_set(got) == want
 
 
+def test_walk_across_fused_step_bound_s32():
+    """As above at s = 32, where the bound is lo32(p) + hi32(p) < 2^32: near
+    2^56 the failing run is the top 2^24 of a 2^32 block, starting at
+    R = 2^56 + 2^32 - 2^24.  The range covers 2^21 of primes below R and
+    2^22 above it; kmax 2^25 - 1 keeps s at 32."""
+    R = 2**56 + 2**32 - 2**24
+    kw = dict(pmin=R - 2**21, pmax=R + 2**22, kmin=3, kmax=2**25 - 1, nmin=100000, nmax=1100000, sign=1)
+    params = ppsd.derive(ppsd.SearchParams(**kw))
+    pr = orc.derive(**kw)
+    P = orc.primes(kw["pmin"], kw["pmax"])
+    slow = sum(1 for p in P.tolist() if (p & 0xFFFFFFFF) + (p >> 32) >= 2**32)
+    assert params.nstep == 32 and 50000 < slow < len(P) - 30000, (params.nstep, slow, len(P))
+    want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=16))
+    assert len(want_raw) > 20
+    with ppsd.Context(params) as ctx:
+        raw, cnt, _ = ctx.run_range(kw["pmin"], kw["pmax"], raw=True)
+        braw, _, bcnt, _ = ctx.batch_run([params], raw=True)
+    assert cnt == len(P) and int(bcnt[0]) == len(P)
+    assert ppsd.factor_set(raw) == want_raw
+    assert ppsd.factor_set(braw) == want_raw
+
+
 def test_many_n_raw_and_filtered():
     """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
     pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
