@@ -81,6 +81,9 @@ int orc_derive(orc_params *pr) {
   if (pr->kmax >= pr->pmin) return fail(pr, ORC_ERR_INVALID_PARAM, "kmax < pmin is required");
   if (pr->kmax - pr->kmin >= (UINT64_C(3) << 36))
     return fail(pr, ORC_ERR_INVALID_PARAM, "kmax-kmin < 3*2^36 is required");
+  /* -n / -N parse as unsigned at most 2^31-1 (pps/app.c:579-580) */
+  if (pr->nmin > 0x7fffffffu || pr->nmax > 0x7fffffffu)
+    return fail(pr, ORC_ERR_INVALID_PARAM, "nmin and nmax must be at most 2^31-1");
   if (pr->nmin == 0) pr->nmin = (uint32_t)(2 * lg2_u64(pr->pmin) - lg2_u64(pr->kmax) - 1);
   if (pr->nmax == 0) pr->nmax = pr->nmin;
   if (pr->nmin > pr->nmax) return fail(pr, ORC_ERR_INVALID_PARAM, "nmin <= nmax is required");

@@ -77,6 +77,9 @@ int ppsd_derive_params(ppsd_params *pr) {
   if (pr->kmin > pr->kmax) return fail(PPSD_ERR_INVALID_PARAM, "kmin <= kmax is required");
   if (pr->kmax >= pr->pmin) return fail(PPSD_ERR_INVALID_PARAM, "kmax < pmin is required");
   if (pr->kmax - pr->kmin >= ((uint64_t)3 << 36)) return fail(PPSD_ERR_INVALID_PARAM, "kmax-kmin < 3*2^36 is required");
+  // -n / -N parse as unsigned at most 2^31-1 (pps/app.c:579-580)
+  if (pr->nmin > 0x7fffffffu || pr->nmax > 0x7fffffffu)
+    return fail(PPSD_ERR_INVALID_PARAM, "nmin and nmax must be at most 2^31-1");
   if (pr->nmin == 0) pr->nmin = (uint32_t)(2 * lg2(pr->pmin) - lg2(pr->kmax) - 1);
   if (pr->nmax == 0) pr->nmax = pr->nmin;
   if (pr->nmin > pr->nmax) return fail(PPSD_ERR_INVALID_PARAM, "nmin <= nmax is required");
@@ -118,10 +121,12 @@ int ppsd_device_info(int device, char *name, uint32_t name_len, uint32_t *cus) {
 
 // What the walk itself needs (ppsd_derive_params gives a subset of this):
 // one candidate per window (kmax * 2^(s-1) < pmin), 2^32 < p < 2^62,
-// nmin >= 64 for the 2^-nmin set-up.
+// nmin >= 64 for the 2^-nmin set-up, nmax <= 2^31-1 so that no window start
+// n + U*s or hit exponent n + i wraps its 32 bits.
 bool ppsd::walk_domain_ok(const ppsd_params *p) {
   return !(p->status != 0 || p->nstep < 1 || p->nstep > 32 || p->pmin <= ((uint64_t)1 << 32) ||
            p->pmax > ((uint64_t)1 << 62) || p->pmin >= p->pmax || p->nmin < 64 || p->nmin > p->nmax ||
+           p->nmax > 0x7fffffffu ||
            p->kmin < 1 || p->kmin > p->kmax || (p->kmax >> (64 - p->nstep)) != 0 ||
            (p->kmax << (p->nstep - 1)) >= p->pmin || p->sign < -1 || p->sign > 1 || p->kstep < 1 ||
            p->koffset >= p->kstep);

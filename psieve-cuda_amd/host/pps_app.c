@@ -17,6 +17,7 @@
 
 #include "../../include/ppsd.h"
 #include "pps_app.h"
+#include "pps_ckpt.h"
 #include "pps_input.h"
 
 uint64_t kmin = 0, kmax = 0;
@@ -333,27 +334,22 @@ void app_thread_fini(int th) {
   fbuf[th] = NULL;
 }
 
-/* Second checkpoint line, "nmin=%u,nmax=%u,factor_count=%u"
- * (pps/app.c:1301-1352). */
+/* The application's checkpoint record, "nmin=..,nmax=..,factor_count=.."
+ * (the line pps/app.c:1348-1352 writes).  It is accepted when it names this
+ * search's exponent range and, if it counts factors already found, the
+ * factors file they were appended to is not empty (pps/app.c:1301-1336
+ * applies the same two conditions); otherwise the count restarts at 0. */
 int app_read_checkpoint(FILE *fin) {
-  unsigned int n0, n1;
-  if (fscanf(fin, "nmin=%u,nmax=%u,factor_count=%u", &n0, &n1, &factor_count) != 3) {
-    factor_count = 0;
+  static const char *const fields[] = {"nmin", "nmax", "factor_count"};
+  uint64_t v[3];
+  struct stat st;
+  factor_count = 0;
+  if (!ckpt_read_record(fin, fields, 3, v) || v[0] != nmin || v[1] != nmax || v[2] > UINT32_MAX) return 0;
+  if (v[2] > 0 && (fstat(fileno(factors_file), &st) != 0 || st.st_size == 0)) {
+    bmsg("Factors file missing.  Restarting test.\n");
     return 0;
   }
-  if (n0 != nmin || n1 != nmax) {
-    factor_count = 0;
-    return 0;
-  }
-  if (factor_count > 0) {
-    struct stat st;
-    fstat(fileno(factors_file), &st);
-    if (st.st_size == 0) {
-      bmsg("Factors file missing.  Restarting test.\n");
-      factor_count = 0;
-      return 0;
-    }
-  }
+  factor_count = (unsigned int)v[2];
   return 1;
 }
 

@@ -26,6 +26,7 @@
 #include <unistd.h>
 
 #include "pps_app.h"
+#include "pps_ckpt.h"
 #include "pps_hostsieve.h"
 
 #define MAX_THREADS 32
@@ -249,38 +250,46 @@ static void process_args(int argc, char *argv[]) {
   }
 }
 
-/* "name=value" or "name value" lines, '#' comments (main.c:578-650). */
+/* One option of ppconfig.txt by its long name; arg is NULL when the line
+ * has no value.  The messages are the reference's (main.c:597-646). */
+static void apply_config_option(const char *fn, const char *name, char *arg) {
+  const struct option *o = long_opts;
+  while (o->name && strcmp(o->name, name) != 0) o++;
+  if (!o->name) {
+    fprintf(stderr, "%s%s: unrecognised option `%s'\n", bmprefix(), fn, name);
+    exit(EXIT_FAILURE);
+  }
+  if (o->has_arg == no_argument && arg) {
+    fprintf(stderr, "%s%s: option `%s' doesn't allow an argument\n", bmprefix(), fn, name);
+    exit(EXIT_FAILURE);
+  }
+  if (o->has_arg == required_argument && !arg) {
+    fprintf(stderr, "%s%s: option `%s' requires an argument\n", bmprefix(), fn, name);
+    exit(EXIT_FAILURE);
+  }
+  const int st = parse_option(o->val, arg, fn);
+  if (st == 0) return;
+  fprintf(stderr, "%s%s: %s argument %s %s\n", bmprefix(), fn,
+          st == -1 ? "invalid" : st == -2 ? "out of range" : "weird", name, arg ? arg : "");
+  exit(EXIT_FAILURE);
+}
+
+/* ppconfig.txt (read before the command line, which overrides it): one
+ * option per line by its long name -- "name=value", "name value" or a bare
+ * "name" -- with blank lines and '#' comment lines skipped. */
 static void read_config_file(const char *fn) {
-  const char *delim = "= \n\r\t\v";
-  char line[128];
+  static const char seps[] = "= \t\r\n\v";
+  char line[256];
   FILE *f = bfopen(fn, "r");
   if (!f) return;
-  while (fgets(line, sizeof line, f) != NULL) {
-    char *str = strtok(line, delim), *arg;
-    int ind;
-    if (str == NULL || str[0] == '#') continue;
-    arg = strtok(NULL, delim);
-    for (ind = 0; long_opts[ind].name; ind++)
-      if (strcmp(str, long_opts[ind].name) == 0) break;
-    if (long_opts[ind].name == NULL) {
-      fprintf(stderr, "%s%s: unrecognised option `%s'\n", bmprefix(), fn, str);
-      exit(EXIT_FAILURE);
-    }
-    if (long_opts[ind].has_arg == no_argument && arg != NULL) {
-      fprintf(stderr, "%s%s: option `%s' doesn't allow an argument\n", bmprefix(), fn, str);
-      exit(EXIT_FAILURE);
-    } else if (long_opts[ind].has_arg == required_argument && arg == NULL) {
-      fprintf(stderr, "%s%s: option `%s' requires an argument\n", bmprefix(), fn, str);
-      exit(EXIT_FAILURE);
-    }
-    switch (parse_option(long_opts[ind].val, arg, fn)) {
-      case 0: break;
-      case -1: fprintf(stderr, "%s%s: invalid argument %s %s\n", bmprefix(), fn, str, arg); exit(EXIT_FAILURE);
-      case -2:
-        fprintf(stderr, "%s%s: out of range argument %s %s\n", bmprefix(), fn, str, arg);
-        exit(EXIT_FAILURE);
-      default: fprintf(stderr, "%s%s: weird argument %s %s\n", bmprefix(), fn, str, arg); exit(EXIT_FAILURE);
-    }
+  while (fgets(line, sizeof line, f)) {
+    char *name = line + strspn(line, " \t\r\n\v");
+    if (*name == '\0' || *name == '#') continue;
+    char *name_end = name + strcspn(name, seps);
+    char *val = name_end + strspn(name_end, seps);
+    val[strcspn(val, " \t\r\n\v")] = '\0';
+    *name_end = '\0';
+    apply_config_option(fn, name, *val ? val : NULL);
   }
   fclose(f);
 }
@@ -320,29 +329,36 @@ static void write_checkpoint(uint64_t p) {
   fclose(fout);
 }
 
-/* main.c:332-372: a checkpoint is used only when its pmin matches, pmin <
- * p < pmax, the application accepts its second line and the checksum holds. */
+/* The checkpoint (format of main.c:305-330) is taken when its first record
+ * names this pmin and a resume point inside (pmin, pmax), the application
+ * accepts the second record and the checksum -- the wrapping sum of pmin,
+ * p, count and sum -- holds (the acceptance rule of main.c:332-372).  A
+ * file left under the reference's older name is read when the current one
+ * is absent. */
 static uint64_t read_checkpoint(void) {
-  uint64_t p0, p, count, sum, checksum;
+  static const char *const fields[] = {"pmin", "p", "count", "sum", "checksum"};
+  enum { F_PMIN, F_P, F_COUNT, F_SUM, F_CHECK, NF };
+  uint64_t v[NF];
   const char *cpf = checkpoint_filename;
-  int valid = 0;
   FILE *fin = fopen(cpf, "r");
+  int ok;
   if (!fin) {
-    if ((fin = fopen(OLD_CHECKPOINT_FILENAME, "r")) == NULL) return pmin;
     cpf = OLD_CHECKPOINT_FILENAME;
+    fin = fopen(cpf, "r");
+    if (!fin) return pmin;
   }
-  if (fscanf(fin, "pmin=%" SCNu64 ",p=%" SCNu64 ",count=%" SCNu64 ",sum=0x%" SCNx64 ",checksum=0x%" SCNx64 "\n", &p0,
-             &p, &count, &sum, &checksum) == 5)
-    if (p0 == pmin && p > pmin && p < pmax) valid = app_read_checkpoint(fin);
+  ok = ckpt_read_record(fin, fields, NF, v) && v[F_PMIN] == pmin && v[F_P] > pmin && v[F_P] < pmax;
+  ok = ok && app_read_checkpoint(fin);
+  ok = ok && v[F_PMIN] + v[F_P] + v[F_COUNT] + v[F_SUM] == v[F_CHECK];
   fclose(fin);
-  if (valid && p0 + p + count + sum == checksum) {
-    fprintf(stderr, "%sResuming from checkpoint p=%" PRIu64 " in %s\n", bmprefix(), p, cpf);
-    cand_count = count;
-    cand_sum = sum;
-    return p;
+  if (!ok) {
+    fprintf(stderr, "%sIgnoring invalid checkpoint in %s\n", bmprefix(), cpf);
+    return pmin;
   }
-  fprintf(stderr, "%sIgnoring invalid checkpoint in %s\n", bmprefix(), cpf);
-  return pmin;
+  fprintf(stderr, "%sResuming from checkpoint p=%" PRIu64 " in %s\n", bmprefix(), v[F_P], cpf);
+  cand_count = v[F_COUNT];
+  cand_sum = v[F_SUM];
+  return v[F_P];
 }
 
 /* Next chunk for a worker, or NULL when done/stopping; honours checkpoint
@@ -480,17 +496,28 @@ static void handle_signal(int signum) {
   stopping = 1;
 }
 
-static void report_status(uint64_t t_now, uint64_t t_start, uint64_t progress, uint64_t last_t, uint64_t last_p,
-                          uint64_t cpu_now, uint64_t cpu_last) {
-  double rate = (double)(progress - last_p) / (double)(t_now - last_t + 1); /* p per usec = M p/sec */
-  const char *unit = "M";
-  int prec;
-  if (rate < 1.0) rate *= 1000.0, unit = "K";
-  if (rate < 1.0) rate *= 1000.0, unit = "";
-  prec = rate < 10.0 ? 3 : rate < 100.0 ? 2 : rate < 1000.0 ? 1 : 0;
-  (void)t_start;
-  printf("p=%" PRIu64 ", %.*f%s p/sec, %.2f CPU cores, %.1f%% done.  ", progress, prec, rate, unit,
-         (double)(cpu_now - cpu_last) / (double)(t_now - last_t + 1), 100.0 * (double)(progress - pmin) / (double)(pmax - pmin));
+/* The status line (main.c:246-300): the frontier p, the p-range rate since
+ * the last report (K/M units, about four significant digits), the CPU cores
+ * the process kept busy meanwhile, the fraction of [pmin, pmax) done, and
+ * the finish time extrapolated from the progress since this run started at
+ * p_resume (strftime "ETA %d %b %H:%M", empty before any progress). */
+static void report_status(uint64_t progress, uint64_t p_resume, time_t run_started, uint64_t dt_usec, uint64_t dp,
+                          uint64_t dcpu_usec) {
+  static const char *const units[] = {"", "K", "M"};
+  double rate = (double)dp / (double)(dt_usec ? dt_usec : 1) * 1e6; /* p per second */
+  int u = 0, digits;
+  char eta[32] = "";
+  while (u < 2 && rate >= 1000.0) rate /= 1000.0, u++;
+  digits = rate < 10.0 ? 3 : rate < 100.0 ? 2 : rate < 1000.0 ? 1 : 0;
+  if (progress > p_resume && pmax > p_resume) {
+    const double frac = (double)(progress - p_resume) / (double)(pmax - p_resume);
+    const time_t finish = run_started + (time_t)((double)(time(NULL) - run_started) / frac);
+    struct tm tmv;
+    if (!localtime_r(&finish, &tmv) || !strftime(eta, sizeof eta, "ETA %d %b %H:%M", &tmv)) eta[0] = '\0';
+  }
+  printf("p=%" PRIu64 ", %.*f%s p/sec, %.2f CPU cores, %.1f%% done. %s  ", progress, digits, rate, units[u],
+         (double)dcpu_usec / (double)(dt_usec ? dt_usec : 1), 100.0 * (double)(progress - pmin) / (double)(pmax - pmin),
+         eta);
   if (quiet_opt) {
     putchar('\r');
     fflush(stdout);
@@ -501,6 +528,7 @@ static void report_status(uint64_t t_now, uint64_t t_start, uint64_t progress, u
 int main(int argc, char *argv[]) {
   const uint64_t program_start = now_usec();
   uint64_t sieve_start, cpu_start, pstop, last_ckpt, last_rep, last_rep_p, last_rep_cpu;
+  time_t run_started;
   pthread_t tid[MAX_THREADS];
   int th, process_ret = EXIT_SUCCESS;
 
@@ -553,6 +581,7 @@ int main(int argc, char *argv[]) {
   signal(SIGHUP, handle_signal);
 
   sieve_start = now_usec();
+  run_started = time(NULL);
   cpu_start = cpu_usec();
   last_ckpt = last_rep = sieve_start;
   last_rep_p = pstart;
@@ -566,7 +595,7 @@ int main(int argc, char *argv[]) {
     uint64_t t = now_usec(), wake;
     if (report_opt && t >= last_rep + (uint64_t)report_opt * 1000000u) {
       uint64_t cpu = cpu_usec();
-      report_status(t, sieve_start, progress_p(), last_rep, last_rep_p, cpu, last_rep_cpu);
+      report_status(progress_p(), pstart, run_started, t - last_rep, progress_p() - last_rep_p, cpu - last_rep_cpu);
       last_rep = t;
       last_rep_p = progress_p();
       last_rep_cpu = cpu;

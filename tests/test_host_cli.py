@@ -118,3 +118,49 @@ def test_no_device_fails(wd):
     rc, _, err = run(MANY_N, wd)
     assert rc == 1
     assert "not usable" in err
+
+
+def _ckpt_line(pmin, p, count, s, fmt_sum="0x{:016x}"):
+    return f"pmin={pmin},p={p},count={count},sum={fmt_sum.format(s)},checksum=0x{pmin + p + count + s:016x}\n"
+
+
+@pytest.mark.parametrize("first,second,resumes", [
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10), "nmin=76,nmax=2000000,factor_count=0\n", True),
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10, "0x{:X}"), "nmin=76,nmax=2000000,factor_count=0\n", True),
+    # fields out of order, missing, or trailing text: not a checkpoint record
+    ("p=42070000500001,pmin=42070000000000,count=5,sum=0xa,checksum=0x0\n", "nmin=76,nmax=2000000,factor_count=0\n",
+     False),
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10).replace(",count=5", ""),
+     "nmin=76,nmax=2000000,factor_count=0\n", False),
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10)[:-1] + ",x=1\n", "nmin=76,nmax=2000000,factor_count=0\n",
+     False),
+    # the application's record names another exponent range, or is cut short
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10), "nmin=77,nmax=2000000,factor_count=0\n", False),
+    (_ckpt_line(42070000000000, 42070000500001, 5, 10), "nmin=76,nmax=2000000\n", False),
+    # resume point outside (pmin, pmax)
+    (_ckpt_line(42070000000000, 42070000000000, 5, 10), "nmin=76,nmax=2000000,factor_count=0\n", False),
+    (_ckpt_line(42070000000000, 42070050000000, 5, 10), "nmin=76,nmax=2000000,factor_count=0\n", False),
+])
+def test_checkpoint_records(wd, first, second, resumes):
+    """The two checkpoint records are read field by field in the reference's
+    format (main.c:305-372, pps/app.c:1301-1352): anything but the exact
+    fields in order is ignored, as is a record for other parameters."""
+    with open(os.path.join(wd, "ppcheck42070e9.txt"), "w") as f:
+        f.write(first + second)
+    _, _, err = run(MANY_N, wd)
+    if resumes:
+        assert "Resuming from checkpoint p=42070000500001 in ppcheck42070e9.txt" in err
+    else:
+        assert "Ignoring invalid checkpoint in ppcheck42070e9.txt" in err
+
+
+def test_config_file_spacing(wd):
+    """Blank lines, leading blanks, tabs and '=' with blanks around it."""
+    with open(os.path.join(wd, "ppconfig.txt"), "w") as f:
+        f.write("\n   # indented comment\n\tpmin = 42070e9\npmax\t42070050e6\n  kmin=1201\nkmax =9999\n\nnmax= 2M\n")
+    _, out, err = run([], wd)
+    assert "ppsieve initialized: 1201 <= k <= 9999, 76 <= n < 2000000" in out, out + err
+    with open(os.path.join(wd, "ppconfig.txt"), "w") as f:
+        f.write("pmin=42070e9\nkmax=9x9\n")
+    rc, _, err = run([], wd)
+    assert rc == 1 and "invalid argument kmax 9x9" in err

@@ -9,11 +9,21 @@ import pytest
 import pps_oracle as orc
 from conftest import REPO
 
-EXE = os.path.join(REPO, "psieve-cuda_amd", "host", "hostsieve_check")
+HOST = os.path.join(REPO, "psieve-cuda_amd", "host")
 
 
-def sieve(ranges, threads=1):
-    args = [EXE, f"-t{threads}"] + [str(x) for r in ranges for x in r]
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    """The check driver (tests/hostsieve_check.c) built against the host sieve."""
+    out = str(tmp_path_factory.mktemp("hostsieve") / "hostsieve_check")
+    subprocess.run(["gcc", "-O3", "-Wall", "-std=gnu11", "-I", HOST, "-o", out,
+                    os.path.join(REPO, "tests", "hostsieve_check.c"), os.path.join(HOST, "pps_hostsieve.c"), "-lm",
+                    "-lpthread"], check=True)
+    return out
+
+
+def sieve(exe, ranges, threads=1):
+    args = [exe, f"-t{threads}"] + [str(x) for r in ranges for x in r]
     out = subprocess.run(args, capture_output=True, text=True, timeout=300, check=True).stdout.split("\n")
     res = []
     for line in out:
@@ -31,23 +41,23 @@ def sieve(ranges, threads=1):
     (10**12 + 1, 10**12 + 2),                       # one candidate
     (10**12 + 6, 10**12 + 7),                       # none
 ])
-def test_ranges_vs_oracle(p0, p1):
-    (cnt, s, order), = sieve([(p0, p1)])
+def test_ranges_vs_oracle(exe, p0, p1):
+    (cnt, s, order), = sieve(exe, [(p0, p1)])
     want = orc.count_primes(p0, p1)
     assert (cnt, s) == (want[0], want[1] % 2**64)
     assert order == "ascending"
 
 
-def test_threads_split_the_range():
+def test_threads_split_the_range(exe):
     """2^28 numbers at 2^40 in 4 parts (parts are at least 2^26 numbers)."""
     r = (2**40 + 12345, 2**40 + 2**28 + 12345)
     want = orc.count_primes(*r)
     for t in (1, 4, 7):
-        (cnt, s, order), = sieve([r], threads=t)
+        (cnt, s, order), = sieve(exe, [r], threads=t)
         assert (cnt, s, order) == (want[0], want[1] % 2**64, "ascending"), t
 
 
-def test_small_prime_table():
+def test_small_prime_table(exe):
     """Primes below 10^6 from 3: pi(10^6) - 1 of them."""
-    (cnt, _, _), = sieve([(3, 10**6)])
+    (cnt, _, _), = sieve(exe, [(3, 10**6)])
     assert cnt == 78497

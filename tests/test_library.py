@@ -78,3 +78,18 @@ def test_collect_without_submit_is_an_error():
     rc = ppsd.lib().ppsd_collect(None, None, 0, None, None, None)
     assert rc == ppsd.PPSD_ERR_INVALID_PARAM
     assert ppsd.lib().ppsd_close(None) == 0
+
+
+def test_derive_rejects_exponents_past_31_bits():
+    """-n/-N parse at most 2^31-1 (pps/app.c:579-580); the library rejects
+    more instead of letting a window start or hit exponent wrap 32 bits."""
+    base = dict(pmin=2**50, pmax=2**50 + 10**6, kmin=3, kmax=9999, nmin=100)
+    _same(dict(base, nmax=2**31 - 1))
+    for nmax in (2**31, 2**32 - 1):
+        _same(dict(base, nmax=nmax))
+        with pytest.raises(ppsd.PPSDError) as e:
+            ppsd.derive(ppsd.SearchParams(**dict(base, nmax=nmax)))
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+    p = ppsd.manual_params(2**50, 2**50 + 10**6, 3, 9999, 100, 2**31, nstep=32)
+    h = ctypes.c_void_p()
+    assert ppsd.lib().ppsd_open(0, ctypes.byref(p), 0, ctypes.byref(h)) == ppsd.PPSD_ERR_INVALID_PARAM
