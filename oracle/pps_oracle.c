@@ -28,6 +28,7 @@
 #include "pps_oracle.h"
 
 #include <inttypes.h>
+#include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -194,8 +195,47 @@ static uint32_t small_sieve(uint32_t lim, uint32_t **out) {
  * most cap of them.  Candidates are the odd p in [p0|1, p1) as in
  * sieve.c:729-760; composites are removed with every odd prime
  * q <= isqrt(p1-1) (the reference uses (unsigned)sqrt((double)pmax),
- * main.c:971-972).  Small windows high up use Miller-Rabin instead, which
- * yields the same set. */
+ * main.c:971-972).  A window that is narrow for its height is instead
+ * presieved with the odd primes below 2^16 and its survivors tested by
+ * Miller-Rabin, which yields the same set; the choice is by estimated
+ * cost (a remainder per sieving prime per segment against about 20 us per
+ * prime tested). */
+static pthread_mutex_t base_mu = PTHREAD_MUTEX_INITIALIZER;
+static uint32_t *base_q, base_n, base_lim;
+static const uint32_t *base_primes(uint32_t lim, uint32_t *n);
+
+static int64_t primes_mr(uint64_t lo, uint64_t p1, uint64_t *out, uint64_t cap, uint64_t n) {
+  uint32_t nq;
+  const uint32_t *qs = base_primes(65536, &nq);
+  const uint64_t SEG = UINT64_C(1) << 18;
+  uint8_t *seg = malloc(SEG);
+  for (uint64_t base = lo; base < p1; base += 2 * SEG) {
+    uint64_t cnt = (p1 - base + 1) / 2;
+    if (cnt > SEG) cnt = SEG;
+    memset(seg, 0, cnt);
+    for (uint32_t qi = 0; qi < nq; qi++) {
+      const uint64_t q = qs[qi];
+      if (q >= 65536 || q * q >= base + 2 * cnt) break;
+      uint64_t first = q * q;
+      if (first < base) {
+        const uint64_t r = base % q;
+        first = r ? base + (q - r) : base;
+        if ((first & 1) == 0) first += q;
+      }
+      for (uint64_t m = first; m < base + 2 * cnt; m += 2 * q) seg[(m - base) / 2] = 1;
+    }
+    for (uint64_t j = 0; j < cnt; j++) {
+      const uint64_t x = base + 2 * j;
+      if (!seg[j] && x < p1 && orc_is_prime(x)) {
+        if (n < cap) out[n] = x;
+        n++;
+      }
+    }
+  }
+  free(seg);
+  return (int64_t)n;
+}
+
 int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
   uint64_t lim, n = 0, lo;
   if (p1 <= p0) return 0;
@@ -207,16 +247,15 @@ int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
   if (lo < 3) lo = 3;
   if (lo >= p1) return (int64_t)n;
   lim = orc_isqrt(p1 - 1);
-  if ((p1 - lo) < lim / 8 || lim > (UINT64_C(1) << 31)) {
-    for (uint64_t x = lo; x < p1; x += 2)
-      if (orc_is_prime(x)) {
-        if (n < cap) out[n] = x;
-        n++;
-      }
-    return (int64_t)n;
+  {
+    const double odds = (double)(p1 - lo) / 2, nq = (double)lim / (log((double)lim + 2) + 1);
+    const double sieve_cost = (floor(odds / (1 << 20)) + 1) * nq * 6e-9 + (base_lim >= lim ? 0 : (double)lim * 2e-9);
+    const double mr_cost = 2 * odds / log((double)p1) * 20e-6 + odds * 0.15 * 1.6e-6;
+    if (lim > 0x7fffffffu || mr_cost < sieve_cost) return primes_mr(lo, p1, out, cap, n);
   }
   {
-    uint32_t *qs, nq = small_sieve((uint32_t)lim, &qs);
+    uint32_t nq;
+    const uint32_t *qs = base_primes((uint32_t)lim, &nq);
     const uint64_t SEG = UINT64_C(1) << 20; /* odd numbers per segment */
     uint8_t *seg = malloc(SEG);
     for (uint64_t base = lo; base < p1; base += 2 * SEG) {
@@ -244,34 +283,62 @@ int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
         }
     }
     free(seg);
-    free(qs);
   }
   return (int64_t)n;
 }
 
-/* Count and sum (mod 2^64) of the primes in [p0, p1), p0 >= 3, for ranges
- * too large to list: segments of 2^27 odd numbers, sieved in parallel by
- * `threads` threads with every odd prime <= isqrt(p1-1).  The same set as
- * orc_primes (the reference's count= and sum=, main.c:1232). */
+/* Count and sum (mod 2^64) of the primes in each of several ranges
+ * [p0[i], p1[i]), p0 >= 3, for ranges too large to list: every range is cut
+ * into segments of 2^27 odd numbers, and `threads` threads take (range,
+ * segment) jobs in turn, sieving each with every odd prime <= isqrt of the
+ * range's end.  The same set as orc_primes (the reference's count= and sum=,
+ * main.c:1232).  The odd primes up to the largest isqrt seen are kept for
+ * later calls (they take seconds to make near 2^62). */
+
+static const uint32_t *base_primes(uint32_t lim, uint32_t *n) {
+  pthread_mutex_lock(&base_mu);
+  if (!base_q || lim > base_lim) {
+    uint32_t *q;
+    uint32_t cnt = small_sieve(lim, &q);
+    free(base_q);
+    base_q = q;
+    base_n = cnt;
+    base_lim = lim;
+  }
+  *n = base_n;
+  pthread_mutex_unlock(&base_mu);
+  return base_q;
+}
+
 struct count_job {
-  uint64_t lo, p1, seg;
+  const uint64_t *lo, *p1;
+  uint32_t nranges;
+  uint64_t seg;
   uint32_t nq;
   const uint32_t *qs;
-  uint64_t next, count, sum;
+  uint32_t cur;      /* range of the next job */
+  uint64_t next;     /* start of the next job in that range */
+  uint64_t *count, *sum;
   pthread_mutex_t mu;
 };
 
 static void *count_worker(void *arg) {
   struct count_job *J = arg;
   uint8_t *buf = malloc(J->seg);
-  uint64_t count = 0, sum = 0;
   for (;;) {
+    uint32_t ri;
+    uint64_t base;
     pthread_mutex_lock(&J->mu);
-    const uint64_t base = J->next;
+    while (J->cur < J->nranges && J->next >= J->p1[J->cur]) {
+      if (++J->cur < J->nranges) J->next = J->lo[J->cur];
+    }
+    ri = J->cur;
+    base = J->next;
     J->next += 2 * J->seg;
     pthread_mutex_unlock(&J->mu);
-    if (base >= J->p1) break;
-    uint64_t cnt = (J->p1 - base + 1) / 2;
+    if (ri >= J->nranges) break;
+    const uint64_t p1 = J->p1[ri];
+    uint64_t cnt = (p1 - base + 1) / 2, count = 0, sum = 0;
     if (cnt > J->seg) cnt = J->seg;
     const uint64_t end = base + 2 * cnt;
     memset(buf, 0, cnt);
@@ -291,39 +358,52 @@ static void *count_worker(void *arg) {
         count++;
         sum += base + 2 * j;
       }
+    pthread_mutex_lock(&J->mu);
+    J->count[ri] += count;
+    J->sum[ri] += sum;
+    pthread_mutex_unlock(&J->mu);
   }
   free(buf);
-  pthread_mutex_lock(&J->mu);
-  J->count += count;
-  J->sum += sum;
-  pthread_mutex_unlock(&J->mu);
   return NULL;
 }
 
-int orc_count_primes(uint64_t p0, uint64_t p1, int threads, uint64_t *count, uint64_t *sum) {
+int orc_count_primes_many(uint32_t n, const uint64_t *p0, const uint64_t *p1, int threads, uint64_t *count,
+                          uint64_t *sum) {
   struct count_job J;
   pthread_t th[64];
-  if (p0 < 3 || p1 <= p0 || threads < 1 || threads > 64) return -1;
-  memset(&J, 0, sizeof J);
-  J.lo = p0 | 1;
-  J.p1 = p1;
-  J.seg = UINT64_C(1) << 27;
-  J.next = J.lo;
-  {
-    uint64_t lim = orc_isqrt(p1 - 1);
-    uint32_t *qs;
-    if (lim > (UINT64_C(1) << 32) - 1) return -1;
-    J.nq = small_sieve((uint32_t)lim, &qs);
-    J.qs = qs;
+  uint64_t *lo, top = 0;
+  if (threads < 1 || threads > 64) return -1;
+  for (uint32_t i = 0; i < n; i++) {
+    if (p0[i] < 3 || p1[i] <= p0[i]) return -1;
+    if (p1[i] > top) top = p1[i];
   }
+  if (n == 0) return 0;
+  if (orc_isqrt(top - 1) > (UINT64_C(1) << 32) - 1) return -1;
+  lo = malloc(sizeof(uint64_t) * n);
+  for (uint32_t i = 0; i < n; i++) {
+    lo[i] = p0[i] | 1;
+    count[i] = sum[i] = 0;
+  }
+  memset(&J, 0, sizeof J);
+  J.lo = lo;
+  J.p1 = p1;
+  J.nranges = n;
+  J.seg = UINT64_C(1) << 27;
+  J.qs = base_primes((uint32_t)orc_isqrt(top - 1), &J.nq);
+  J.cur = 0;
+  J.next = lo[0];
+  J.count = count;
+  J.sum = sum;
   pthread_mutex_init(&J.mu, NULL);
   for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, count_worker, &J);
   for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
   pthread_mutex_destroy(&J.mu);
-  free((void *)J.qs);
-  *count = J.count;
-  *sum = J.sum;
+  free(lo);
   return 0;
+}
+
+int orc_count_primes(uint64_t p0, uint64_t p1, int threads, uint64_t *count, uint64_t *sum) {
+  return orc_count_primes_many(1, &p0, &p1, threads, count, sum);
 }
 
 /* ------------------------------------------------------------------ */
@@ -553,6 +633,65 @@ int64_t orc_run_range(const orc_params *pr, uint64_t p0, uint64_t p1, orc_hit *o
 }
 
 /* "%llu | %llu*2^%u%+d\n" as report_factor writes it (app.c:134) */
+/* Checks of reported divisors, independent of the walk: for each hit, bit
+ * ORC_BAD_PRIME if p is not prime (Miller-Rabin), ORC_BAD_DIVIDES if
+ * k*2^n + c is not 0 mod p (square-and-multiply in 128-bit arithmetic),
+ * ORC_BAD_K if k is not odd in [kmin, kmax], ORC_BAD_N if n is outside
+ * [nmin, nmax), ORC_BAD_SIGN if c is not the search's sign (+1 or -1 for
+ * both), and ORC_KEPT if orc_test_factor keeps it.  Returns the number of
+ * hits with any ORC_BAD_* bit. */
+struct check_job {
+  const orc_params *pr;
+  const orc_hit *h;
+  uint64_t lo, hi, bad;
+  uint8_t *flags;
+};
+
+static void *check_thread(void *arg) {
+  struct check_job *J = arg;
+  const orc_params *pr = J->pr;
+  for (uint64_t i = J->lo; i < J->hi; i++) {
+    const orc_hit *x = &J->h[i];
+    uint8_t f = 0;
+    if (!orc_is_prime(x->p)) f |= ORC_BAD_PRIME;
+    if (x->p > 2) {
+      const uint64_t t = mulmod64(x->k % x->p, powmod64(2, x->n, x->p), x->p);
+      const uint64_t want = x->c == 1 ? x->p - 1 : 1; /* k*2^n == -c (mod p) */
+      if (t != want) f |= ORC_BAD_DIVIDES;
+    } else
+      f |= ORC_BAD_DIVIDES;
+    if (!(x->k & 1) || x->k < pr->kmin || x->k > pr->kmax) f |= ORC_BAD_K;
+    if (x->n < pr->nmin || x->n >= pr->nmax) f |= ORC_BAD_N;
+    if (!((x->c == 1 || x->c == -1) && (pr->sign == 0 || x->c == pr->sign))) f |= ORC_BAD_SIGN;
+    if (f) J->bad++;
+    if (!f && orc_test_factor(pr, x->k, x->n, x->c)) f |= ORC_KEPT;
+    J->flags[i] = f;
+  }
+  return NULL;
+}
+
+int64_t orc_check_hits(const orc_params *pr, const orc_hit *h, uint64_t n, int nthreads, uint8_t *flags) {
+  struct check_job jobs[64];
+  pthread_t tid[64];
+  int64_t bad = 0;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 64) nthreads = 64;
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t].pr = pr;
+    jobs[t].h = h;
+    jobs[t].lo = n * t / nthreads;
+    jobs[t].hi = n * (t + 1) / nthreads;
+    jobs[t].bad = 0;
+    jobs[t].flags = flags;
+    pthread_create(&tid[t], NULL, check_thread, &jobs[t]);
+  }
+  for (int t = 0; t < nthreads; t++) {
+    pthread_join(tid[t], NULL);
+    bad += (int64_t)jobs[t].bad;
+  }
+  return bad;
+}
+
 int orc_format_hit(const orc_hit *h, char *buf, int len) {
   return snprintf(buf, (size_t)len, "%" PRIu64 " | %" PRIu64 "*2^%u%+d\n", h->p, h->k, h->n, h->c);
 }

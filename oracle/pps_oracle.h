@@ -45,6 +45,8 @@ uint64_t orc_isqrt(uint64_t x);
 int orc_is_prime(uint64_t n);
 int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap);
 int orc_count_primes(uint64_t p0, uint64_t p1, int threads, uint64_t *count, uint64_t *sum);
+int orc_count_primes_many(uint32_t n, const uint64_t *p0, const uint64_t *p1, int threads, uint64_t *count,
+                          uint64_t *sum);
 uint64_t orc_neg_inv64(uint64_t p);
 uint64_t orc_inv2pow(uint64_t p, uint32_t n);
 uint64_t orc_step(uint64_t K, uint64_t p, uint64_t ps, uint32_t s);
@@ -55,6 +57,13 @@ int64_t orc_walk(const orc_params *pr, const uint64_t *P, uint64_t np,
 int64_t orc_run_range(const orc_params *pr, uint64_t p0, uint64_t p1,
                       orc_hit *out, uint64_t cap, int filter, int nthreads,
                       uint64_t *count, uint64_t *sum);
+#define ORC_BAD_PRIME 1
+#define ORC_BAD_DIVIDES 2
+#define ORC_BAD_K 4
+#define ORC_BAD_N 8
+#define ORC_BAD_SIGN 16
+#define ORC_KEPT 32
+int64_t orc_check_hits(const orc_params *pr, const orc_hit *h, uint64_t n, int nthreads, uint8_t *flags);
 int orc_format_hit(const orc_hit *h, char *buf, int len);
 
 #ifdef __cplusplus

@@ -56,6 +56,8 @@ def lib():
         L.orc_primes.restype = C.c_int64
         L.orc_count_primes.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _u64p, _u64p]
         L.orc_count_primes.restype = C.c_int
+        L.orc_count_primes_many.argtypes = [C.c_uint32, _u64p, _u64p, C.c_int, _u64p, _u64p]
+        L.orc_count_primes_many.restype = C.c_int
         L.orc_neg_inv64.argtypes = [C.c_uint64]
         L.orc_neg_inv64.restype = C.c_uint64
         L.orc_inv2pow.argtypes = [C.c_uint64, C.c_uint32]
@@ -66,6 +68,8 @@ def lib():
         L.orc_try_all_factors.restype = C.c_int
         L.orc_test_factor.argtypes = [P, C.c_uint64, C.c_uint32, C.c_int]
         L.orc_test_factor.restype = C.c_int
+        L.orc_check_hits.argtypes = [P, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
+        L.orc_check_hits.restype = C.c_int64
         L.orc_walk.argtypes = [P, _u64p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_int, C.c_int]
         L.orc_walk.restype = C.c_int64
         L.orc_run_range.argtypes = [P, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
@@ -97,6 +101,41 @@ def count_primes(p0: int, p1: int, threads: int = 8):
     if rc:
         raise ValueError("orc_count_primes: range outside its domain")
     return c.value, s.value
+
+
+def count_primes_many(ranges, threads: int = 8):
+    """[(count, sum mod 2^64)] of the primes in each [p0, p1) of ranges,
+    sieved together (one table of sieving primes for all)."""
+    n = len(ranges)
+    p0 = np.array([r[0] for r in ranges], dtype=np.uint64)
+    p1 = np.array([r[1] for r in ranges], dtype=np.uint64)
+    c = np.zeros(n, dtype=np.uint64)
+    s = np.zeros(n, dtype=np.uint64)
+    if n and lib().orc_count_primes_many(n, p0.ctypes.data_as(_u64p), p1.ctypes.data_as(_u64p), threads,
+                                         c.ctypes.data_as(_u64p), s.ctypes.data_as(_u64p)):
+        raise ValueError("orc_count_primes_many: a range outside its domain")
+    return [(int(a), int(b)) for a, b in zip(c, s)]
+
+
+def test_factor_mask(pr: OrcParams, hits) -> np.ndarray:
+    """orc_test_factor (pps/app.c:150-186) of every (k, n, c) of hits."""
+    f = lib().orc_test_factor
+    return np.array([f(C.byref(pr), int(h["k"]), int(h["n"]), int(h["c"])) != 0 for h in hits], dtype=bool)
+
+
+BAD_PRIME, BAD_DIVIDES, BAD_K, BAD_N, BAD_SIGN, KEPT = 1, 2, 4, 8, 16, 32
+
+
+def check_hits(pr: OrcParams, hits: np.ndarray, threads: int = 8) -> np.ndarray:
+    """Per hit (HIT_DTYPE): the BAD_* bits of what is wrong with it as a
+    divisor of the search pr (p not prime, not a divisor, k, n or sign out of
+    range), or KEPT when it is valid and test_factor keeps it."""
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    flags = np.zeros(len(hits), dtype=np.uint8)
+    if len(hits):
+        lib().orc_check_hits(C.byref(pr), hits.ctypes.data_as(C.c_void_p), len(hits), threads,
+                             flags.ctypes.data_as(C.c_void_p))
+    return flags
 
 
 def walk(pr: OrcParams, P: np.ndarray, filter: bool = True, threads: int = 1) -> np.ndarray:

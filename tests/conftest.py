@@ -24,3 +24,13 @@ def golden_dir():
 def read_lines(name):
     with open(os.path.join(GOLDEN, name)) as f:
         return [l.rstrip("\n") for l in f if l.strip()]
+
+
+def windowed_run(tag):
+    """(params, windows) of a windowed golden run: the search's parameters
+    and the p-windows [a, b) whose primes the oracle walked completely."""
+    import json
+
+    with open(os.path.join(GOLDEN, "runs.json")) as f:
+        r = json.load(f)[tag]
+    return dict(r["params"]), [tuple(w) for w in r["windows"]]

@@ -16,7 +16,10 @@ Fixtures computed by the oracle and cross-checked here before writing:
   many_n_raw.txt         every divisor of the many_n_test range (no filter);
                          its 3/5/7-free subset must equal oldsort.txt exactly
   cfg1_factors.txt       factors of -p1e12 -P1000100M -k3 -K9999 -n100 -N200000
-  cfg2_slice.txt / cfg3_slice.txt   factors of p-slices at the 2^50 / 2^61 sizes
+  cfg2_windows[_raw].txt / cfg3_windows[_raw].txt
+                         factors (and raw divisors) of p-windows of the 2^50
+                         and 2^61 (both signs) searches, each window around a
+                         divisor (see windows())
   runs.json              prime count/sum and factor counts of each oracle run
 """
 import json
@@ -123,13 +126,79 @@ def main():
 
     write_lines("cfg1_factors.txt", run("cfg1", pmin=10**12, pmax=10**12 + 10**8, kmin=3, kmax=9999, nmin=100,
                                         nmax=200000))
-    write_lines("cfg2_slice.txt", run("cfg2_slice", pmin=2**50, pmax=2**50 + 3 * 10**6, kmin=1201, kmax=9999,
-                                      nmin=50000, nmax=6000000))
-    write_lines("cfg3_slice.txt", run("cfg3_slice", pmin=2**61, pmax=2**61 + 10**6, kmin=3, kmax=100001,
-                                      nmin=100000, nmax=10000000, sign=0))
     with open(os.path.join(HERE, "runs.json"), "w") as f:
         json.dump(runs, f, indent=1, sort_keys=True)
-    print(json.dumps(runs, indent=1))
+    windows()
+
+
+# Window centres: p-values of divisors that a device run over the whole
+# 2^50 search (24 of its 3601 factors, evenly spaced) and over the whole 2^61
+# both-signs search (all 22 of its raw divisors) reported.  They only place
+# the windows, so that the goldens hold hits (a contiguous slice of either
+# search the oracle can walk holds next to none); what each window contains
+# is decided by the oracle's complete walk of every prime in it.
+CFG2_CENTRES = [1125901037288939, 1125903237675401, 1125905246522519, 1125907220481349, 1125909255010373,
+                1125911391247831, 1125913426274423, 1125915649379629, 1125917588337161, 1125919643038243,
+                1125921460578587, 1125923800585219, 1125926030070847, 1125928229982109, 1125930444025969,
+                1125932566212599, 1125934883685323, 1125936917007559, 1125939040888273, 1125940925008381,
+                1125942897869017, 1125945114077401, 1125947009987603, 1125948965101183]
+CFG3_CENTRES = [2305843009378763033, 2305843009432867523, 2305843009554438901, 2305843009571129923,
+                2305843009741738669, 2305843009813309303, 2305843009835655683, 2305843010018116507,
+                2305843010100330797, 2305843010303526457, 2305843010330208617, 2305843010377028831,
+                2305843010400013697, 2305843010443865203, 2305843010487221651, 2305843010568480431,
+                2305843010612587969, 2305843010809514747, 2305843010938955149, 2305843011046819063,
+                2305843011047201137, 2305843011190992643]
+WINDOWED = {
+    "cfg2_windows": (dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000),
+                     CFG2_CENTRES, 350000),
+    "cfg3_windows": (dict(pmin=2**61, pmax=2**61 + 2 * 10**9, kmin=3, kmax=100001, nmin=100000, nmax=10000000,
+                          sign=0), CFG3_CENTRES, 200000),
+}
+
+
+def merged_windows(centres, half, lo, hi):
+    """[c - half, c + half) around each centre, clipped to [lo, hi), with
+    overlapping windows merged."""
+    out = []
+    for c in sorted(centres):
+        a, b = max(lo, c - half), min(hi, c + half)
+        if out and a <= out[-1][1]:
+            out[-1][1] = max(out[-1][1], b)
+        else:
+            out.append([a, b])
+    return out
+
+
+def windows():
+    """Oracle runs over p-windows of the full 2^50 and 2^61 searches:
+    <tag>_raw.txt every divisor, <tag>.txt the factors test_factor keeps,
+    and in runs.json the windows, their prime count and sum."""
+    path = os.path.join(HERE, "runs.json")
+    with open(path) as f:
+        runs = json.load(f)
+    runs.pop("cfg2_slice", None)
+    runs.pop("cfg3_slice", None)
+    for tag, (kw, centres, half) in WINDOWED.items():
+        pr = orc.derive(**kw)
+        assert pr.status == 0, pr.msg
+        wins = merged_windows(centres, half, kw["pmin"], kw["pmax"])
+        raw, kept, cnt, psum = set(), set(), 0, 0
+        for a, b in wins:
+            P = orc.primes(a, b)
+            raw |= orc.hit_set(orc.walk(pr, P, filter=False, threads=8))
+            kept |= orc.hit_set(orc.walk(pr, P, filter=True, threads=8))
+            cnt += len(P)
+            psum += int(P.sum(dtype="uint64"))
+        for p, k, n, c in raw:
+            assert (k * pow(2, n, p) + c) % p == 0 and sympy.isprime(p)
+        assert kept <= raw and len(kept) > 0 and len(raw) >= 20, (tag, len(kept), len(raw))
+        write_lines(tag + "_raw.txt", sorted(fmt(t) for t in raw))
+        write_lines(tag + ".txt", sorted(fmt(t) for t in kept))
+        runs[tag] = {"params": kw, "windows": wins, "nstep": pr.nstep, "count": cnt, "sum": psum & (2**64 - 1),
+                     "lines": len(kept), "raw_lines": len(raw)}
+        print(tag, len(wins), "windows", cnt, "primes", len(raw), "divisors", len(kept), "factors", flush=True)
+    with open(path, "w") as f:
+        json.dump(runs, f, indent=1, sort_keys=True)
 
 
 FULL = {
@@ -158,5 +227,7 @@ def full_counts():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--full-counts"]:
         full_counts()
+    elif sys.argv[1:] == ["--windows"]:
+        windows()
     else:
         main()

@@ -10,7 +10,7 @@ import pytest
 
 import pps_oracle as orc
 import ppsd
-from conftest import GOLDEN, read_lines
+from conftest import GOLDEN, read_lines, windowed_run
 from unitgen import _dense_unit, _oracle, _ref_unit
 
 pytestmark = pytest.mark.gpu
@@ -60,22 +60,35 @@ def test_batch_mix_vs_oracle(seed):
 
 
 def test_batch_golden_ranges_together():
-    """pps/many_n_test.sh's range, the 1e12 search (s = 27) and slices of the
-    2^50 and 2^61 (both signs) searches as four units of one batch: each
-    unit's factors and prime count/sum are its golden run's."""
-    tags = [("many_n", "many_n_filtered.txt"), ("cfg1", "cfg1_factors.txt"), ("cfg2_slice", "cfg2_slice.txt"),
-            ("cfg3_slice", "cfg3_slice.txt")]
-    params = [ppsd.derive(ppsd.SearchParams(**RUNS[t]["params"])) for t, _ in tags]
+    """pps/many_n_test.sh's range, the 1e12 search (s = 27) and the p-windows
+    of the 2^50 and 2^61 (both signs) searches, each window a unit of one
+    batch: the factors (and, raw, the divisors) of each golden run, and its
+    prime count and sum."""
+    units, owner = [], []
+    for tag in ("many_n", "cfg1"):
+        units.append(RUNS[tag]["params"])
+        owner.append(tag)
+    for tag in ("cfg2_windows", "cfg3_windows"):
+        kw, wins = windowed_run(tag)
+        for a, b in wins:
+            units.append(dict(kw, pmin=a, pmax=b))
+            owner.append(tag)
+    params = [ppsd.derive(ppsd.SearchParams(**u)) for u in units]
     with ppsd.Context(params[0]) as ctx:
         out, unit, cnt, sums = ctx.batch_run(params)
-        raw, raw_unit, _, _ = ctx.batch_run(params[:1], raw=True)
-        st = ctx.stats()
-    got = _per_unit(out, unit, len(tags))
-    for i, (t, name) in enumerate(tags):
-        assert got[i] == golden_set(name), t
-        assert (int(cnt[i]), int(sums[i])) == (RUNS[t]["count"], RUNS[t]["sum"]), t
-    assert _per_unit(raw, raw_unit, 1)[0] == golden_set("many_n_raw.txt")
-    assert st.primes == sum(RUNS[t]["count"] for t, _ in tags) + RUNS["many_n"]["count"]
+        raw, raw_unit, rcnt, _ = ctx.batch_run(params, raw=True)
+    got = _per_unit(out, unit, len(units))
+    got_raw = _per_unit(raw, raw_unit, len(units))
+    for tag, name, raw_name in (("many_n", "many_n_filtered.txt", "many_n_raw.txt"), ("cfg1", "cfg1_factors.txt", None),
+                                ("cfg2_windows", "cfg2_windows.txt", "cfg2_windows_raw.txt"),
+                                ("cfg3_windows", "cfg3_windows.txt", "cfg3_windows_raw.txt")):
+        idx = [i for i, o in enumerate(owner) if o == tag]
+        assert set().union(*(got[i] for i in idx)) == golden_set(name), tag
+        if raw_name:
+            assert set().union(*(got_raw[i] for i in idx)) == golden_set(raw_name), tag
+        assert sum(int(cnt[i]) for i in idx) == RUNS[tag]["count"], tag
+        assert sum(int(sums[i]) for i in idx) % 2**64 == RUNS[tag]["sum"], tag
+    assert [int(x) for x in rcnt] == [int(x) for x in cnt]
 
 
 def test_batch_equals_plain_path():

@@ -3,10 +3,12 @@
 batch path (app_thread_fun), and interruption + resume from a checkpoint."""
 import json
 import os
+import re
 import signal
 import subprocess
 import time
 
+import numpy as np
 import pytest
 
 import pps_oracle as orc
@@ -65,35 +67,92 @@ def test_riesel_and_both_vs_oracle(tmp_path, sign_args, sign):
     assert len(lines) > 3
 
 
-def test_interrupt_and_resume(tmp_path):
-    """SIGINT mid-run writes a checkpoint at the last finished chunk; the rerun
-    resumes there (main.c:332-372) and the factor file and count/sum end up
-    exactly as in one uninterrupted run."""
-    args = ["-p1e12", "-P1012e9", "-k3", "-K9999", "-n100", "-N2000000", "-q", "-r", "1", "--chunk", "64M"]
-    full = tmp_path / "full"
-    part = tmp_path / "part"
-    full.mkdir()
-    part.mkdir()
-    rc, _, err_full = run(args, full)
-    assert rc == 0, err_full
-    tail_full = [l for l in err_full.splitlines() if l.startswith("count=")]
+RESUME_ARGS = ["-p1e12", "-P1012e9", "-k3", "-K9999", "-n100", "-N2000000", "-q", "-r", "1", "--chunk", "64M"]
 
-    p = subprocess.Popen([EXE] + args, cwd=part, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+
+def _interrupt_then_resume(args, cwd):
+    """Starts ppsieve, sends SIGINT after its first status line, and runs it
+    again to completion from the checkpoint it wrote.  Returns (checkpoint p,
+    the status line, stderr of the resumed run)."""
+    p = subprocess.Popen([EXE] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                          bufsize=1)
     t0 = time.time()
-    for line in p.stdout:  # the first status report ("p=..., ... p/sec") comes after a second
-        if line.startswith("p=") or "p/sec" in line or time.time() - t0 > 60:
+    status = ""
+    for line in iter(p.stdout.readline, ""):  # the first report ("p=..., ... p/sec") comes after a second
+        if "p/sec" in line or time.time() - t0 > 60:
+            status = line
             break
     p.send_signal(signal.SIGINT)
     _, err1 = p.communicate(timeout=120)
     assert "Sieve incomplete" in err1, err1
-    assert os.path.exists(part / "ppcheck1e12.txt")
-    rc, _, err2 = run(args, part)
+    with open(os.path.join(cwd, "ppcheck1e12.txt")) as f:
+        ck = int(f.readline().split(",")[1].split("=")[1])
+    rc, _, err2 = run(args, cwd)
     assert rc == 0, err2
-    assert "Resuming from checkpoint" in err2
-    assert [l for l in err2.splitlines() if l.startswith("count=")] == tail_full
+    assert f"Resuming from checkpoint p={ck}" in err2
+    return ck, status, err2
+
+
+def _check_resumed_run_vs_oracle(cwd, ck, err):
+    """The resumed run's count=/sum= line is the oracle's sieve of the whole
+    range; every factor line is a valid divisor that test_factor keeps,
+    none appears twice, and around the resume point -- 5e6 on either side,
+    one chunk finished before the interruption and one after -- the lines
+    are exactly the oracle's walk of those primes."""
+    lo, hi = 10**12, 1012 * 10**9
+    c, s = orc.count_primes(lo, hi, threads=16)
+    assert f"count={c},sum=0x{s:016x}" in err
+    lines = factors(cwd)
+    assert len(lines) == len(set(lines)) > 1000
+    pr = orc.derive(pmin=lo, pmax=hi, kmin=3, kmax=9999, nmin=100, nmax=2000000)
+    hits = np.array([orc.parse_line(l) for l in lines], dtype=orc.HIT_DTYPE)
+    flags = orc.check_hits(pr, hits, threads=16)
+    assert np.all(flags == orc.KEPT), "a factor line is not a kept divisor"
+    assert lo < ck < hi
+    a, b = max(lo, ck - 5 * 10**6), min(hi, ck + 5 * 10**6)
+    want = orc.walk(pr, orc.primes(a, b), filter=True, threads=16)
+    got = sorted(l for l in lines if a <= int(l.split()[0]) < b)
+    assert got == sorted(f"{int(x['p'])} | {int(x['k'])}*2^{int(x['n'])}{int(x['c']):+d}" for x in want)
+    assert got
+
+
+def test_interrupt_and_resume(tmp_path):
+    """SIGINT mid-run writes a checkpoint at the last finished chunk; the rerun
+    resumes there (main.c:332-372).  The result is checked against the
+    oracle (count, sum, every line, the lines around the resume point) and
+    equals one uninterrupted run.  The status line carries the ETA field
+    (main.c:276-291)."""
+    full = tmp_path / "full"
+    part = tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    rc, _, err_full = run(RESUME_ARGS, full)
+    assert rc == 0, err_full
+    ck, status, err2 = _interrupt_then_resume(RESUME_ARGS, str(part))
+    assert re.search(r"p=\d+, [\d.]+[KM]? p/sec, [\d.]+ CPU cores, [\d.]+% done\. ETA \d\d \w{3} \d\d:\d\d", status), status
+    _check_resumed_run_vs_oracle(str(part), ck, err2)
+    assert [l for l in err2.splitlines() if l.startswith("count=")] == \
+        [l for l in err_full.splitlines() if l.startswith("count=")]
     assert factors(part) == factors(full)
-    assert len(factors(full)) > 1000
+
+
+def test_two_workers_share_device_0(tmp_path):
+    """-t 2 -d 0: two worker threads, each with its own contexts, take p-chunks
+    of one range from the shared frontier (main.c:1026-1031, pps/app.c:801-805)
+    on the one device: the pps/many_n_test.sh range in 2 Mi chunks gives the
+    golden factors and count/sum, and an interrupted two-worker run resumes
+    to the oracle's result."""
+    rc, out, err = run(MANY_N + ["-q", "-t", "2", "-d", "0", "--chunk", "2m"], tmp_path)
+    assert rc == 0, err
+    assert "Thread 0 starting" in err and "Thread 1 starting" in err
+    assert factors(tmp_path) == sorted(read_lines("many_n_filtered.txt"))
+    r = RUNS["many_n"]
+    assert f"count={r['count']},sum=0x{r['sum']:016x}" in err
+    part = tmp_path / "part"
+    part.mkdir()
+    args = RESUME_ARGS[:-1] + ["16M", "-t", "2", "-d", "0"]
+    ck, _, err2 = _interrupt_then_resume(args, str(part))
+    _check_resumed_run_vs_oracle(str(part), ck, err2)
 
 
 def test_periodic_checkpoint(tmp_path):

@@ -1,5 +1,6 @@
 """Parity of the HIP path (libppsd.so through its C ABI) with the oracle and
 the golden fixtures.  Integer work: every comparison is exact set equality."""
+import ctypes
 import json
 import os
 import random
@@ -10,7 +11,7 @@ import sympy
 
 import pps_oracle as orc
 import ppsd
-from conftest import GOLDEN, read_lines
+from conftest import GOLDEN, read_lines, windowed_run
 
 pytestmark = pytest.mark.gpu
 
@@ -153,6 +154,51 @@ def test_walk_across_fused_step_bound_s32():
     assert ppsd.factor_set(braw) == want_raw
 
 
+_S32_RUN_CACHE = {}
+
+
+def _s32_run_near_2p61(sign):
+    """Parameters and oracle sets for a p-range across the start of the run
+    of primes that fail the three-multiply bound in the 2^32 block right
+    above 2^61 (lo32(p) > 2^32 - 1 - 2^29: the top eighth of the block).
+    kmax = 2^29 - 1 keeps s at 32 (33 before the reference's clamp) and gives
+    each prime about 2.3e-4 divisors per sign over the 2e6 exponents."""
+    if sign not in _S32_RUN_CACHE:
+        R = 2**61 + 2**32 - 2**29
+        kw = dict(pmin=R - 2**21, pmax=R + 2**22, kmin=3, kmax=2**29 - 1, nmin=1000000, nmax=3000000, sign=sign)
+        pr = orc.derive(**kw)
+        P = orc.primes(kw["pmin"], kw["pmax"])
+        want_raw = orc.hit_set(orc.walk(pr, P, filter=False, threads=16))
+        want = {t for t in want_raw if orc.lib().orc_test_factor(ctypes.byref(pr), t[1], t[2], t[3])}
+        _S32_RUN_CACHE[sign] = (kw, P, want_raw, want)
+    return _S32_RUN_CACHE[sign]
+
+
+@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("sign", [1, -1, 0])
+def test_four_multiply_loop_s32_near_2p61(sign, form):
+    """Every s = 32 walk loop on its four-multiply branch, with hits: each
+    sign (+1, -1, both) and each pre-test form forced (1 exact, 2 range, 3
+    odd-low) over a range where about a third of the primes fail the
+    three-multiply bound, so whole waves take the four-multiply loop and
+    others the fused one.  Raw divisors and kept factors as the oracle's."""
+    kw, P, want_raw, want = _s32_run_near_2p61(sign)
+    params = ppsd.derive(ppsd.SearchParams(**kw))
+    slow = sum(1 for p in P.tolist() if (p & 0xFFFFFFFF) + (p >> 32) >= 2**32)
+    assert params.nstep == 32 and params.nstep_ref == 32 and 30000 < slow < len(P) - 30000
+    assert len(want_raw) >= 20 * (2 if sign == 0 else 1) and want
+    if sign == 0:
+        assert {t[3] for t in want_raw} == {1, -1}
+    with ppsd.Context(params) as ctx:
+        ctx.set_pretest(form)
+        assert ctx.pretest()[1] == form
+        raw, cnt, psum = ctx.run_range(kw["pmin"], kw["pmax"], raw=True)
+        got, _, _ = ctx.run_range(kw["pmin"], kw["pmax"])
+    assert (cnt, psum) == (len(P), int(P.sum(dtype=np.uint64)))
+    assert ppsd.factor_set(raw) == want_raw
+    assert ppsd.factor_set(got) == want
+
+
 def test_many_n_raw_and_filtered():
     """pps/many_n_test.sh range: all 296 divisors (whose 3/5/7-free part is
     pps/oldsort.txt) and the 39 factors today's filter keeps; prime count and
@@ -211,12 +257,25 @@ def test_cfg1_full_range():
     assert st.primes == RUNS["cfg1"]["count"]
 
 
-def test_cfg2_slice():
-    _golden_run("cfg2_slice", "cfg2_slice.txt")
-
-
-def test_cfg3_slice_both_signs():
-    _golden_run("cfg3_slice", "cfg3_slice.txt")
+@pytest.mark.parametrize("tag", ["cfg2_windows", "cfg3_windows"])
+def test_windows_of_the_large_searches(tag):
+    """The 2^50 search (k 1201..9999, n to 6e6) and the 2^61 both-signs
+    search (k 3..100001, n to 1e7) over p-windows around divisors: every
+    raw divisor and every kept factor of each window, and the windows' prime
+    count and sum, as the oracle's complete walk of them."""
+    kw, wins = windowed_run(tag)
+    raw, got, cnt, psum = set(), set(), 0, 0
+    with context(**kw) as ctx:
+        for a, b in wins:
+            r, c, s = ctx.run_range(a, b, raw=True)
+            f, _, _ = ctx.run_range(a, b)
+            raw |= ppsd.factor_set(r)
+            got |= ppsd.factor_set(f)
+            cnt += c
+            psum = (psum + s) % 2**64
+    assert raw == golden_set(tag + "_raw.txt") and len(raw) >= 20
+    assert got == golden_set(tag + ".txt") and got
+    assert (cnt, psum) == (RUNS[tag]["count"], RUNS[tag]["sum"])
 
 
 def _random_config(rng):

@@ -28,16 +28,23 @@ def golden_set(name):
 
 
 @pytest.mark.parametrize("mode", FORMS)
-@pytest.mark.parametrize("tag,name", [("cfg2_slice", "cfg2_slice.txt"), ("cfg3_slice", "cfg3_slice.txt"),
+@pytest.mark.parametrize("tag,name", [("cfg2_windows", "cfg2_windows_raw.txt"), ("cfg3_windows", "cfg3_windows_raw.txt"),
                                       ("many_n", "many_n_filtered.txt")])
 def test_golden_ranges_each_form(mode, tag, name):
     r = RUNS[tag]
     kw = dict(r["params"])
+    wins = [tuple(w) for w in r.get("windows", [(kw["pmin"], kw["pmax"])])]
+    raw = name.endswith("_raw.txt")
+    got, cnt, psum = set(), 0, 0
     with ppsd.Context(ppsd.derive(ppsd.SearchParams(**kw))) as ctx:
         ctx.set_pretest(mode)
         assert ctx.pretest()[:2] == (mode, mode)
-        got, cnt, psum = ctx.run_range(kw["pmin"], kw["pmax"])
-    assert ppsd.factor_set(got) == golden_set(name)
+        for a, b in wins:
+            f, c, s = ctx.run_range(a, b, raw=raw)
+            got |= ppsd.factor_set(f)
+            cnt += c
+            psum = (psum + s) % 2**64
+    assert got == golden_set(name) and got
     assert (cnt, psum) == (r["count"], r["sum"])
 
 
@@ -78,8 +85,8 @@ def test_automatic_choice():
     4.2e13 with kmax 9999: 1.2e-3 false alarms per chunk); the exact form
     where neither is (2^46, kmax 1e6); a forced form keeps the tuned chunk."""
     big_k = dict(pmin=2**46, pmax=2**46 + 10**6, kmin=3, kmax=999999, nmin=100000, nmax=2000000)
-    for tag, kw, want in (("cfg3_slice", RUNS["cfg3_slice"]["params"], (0, RANGE, 4)),
-                          ("cfg2_slice", RUNS["cfg2_slice"]["params"], (0, ODDLO, 8)),
+    for tag, kw, want in (("cfg3", RUNS["cfg3_windows"]["params"], (0, RANGE, 4)),
+                          ("cfg2", RUNS["cfg2_windows"]["params"], (0, ODDLO, 8)),
                           ("many_n", RUNS["many_n"]["params"], (0, ODDLO, 8)),
                           ("2^46 kmax 1e6", big_k, (0, EXACT, 8))):
         with ppsd.Context(ppsd.derive(ppsd.SearchParams(**kw))) as ctx:
@@ -88,7 +95,7 @@ def test_automatic_choice():
             for other in FORMS:
                 ctx.set_pretest(other)
                 assert ctx.pretest() == (other, other, 8)
-    with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS["cfg2_slice"]["params"]))) as ctx:
+    with ppsd.Context(ppsd.derive(ppsd.SearchParams(**RUNS["cfg2_windows"]["params"]))) as ctx:
         with pytest.raises(ppsd.PPSDError):
             ctx.set_pretest(4)
         with pytest.raises(ppsd.PPSDError):

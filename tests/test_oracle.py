@@ -227,7 +227,7 @@ def test_golden_factors_reverify_in_reference_factor_proth():
     check: its stdin driver echoes a line with no odd factor below 2^15
     unchanged (pps/factor_proth.c:255-282; the driver handles +1 only)."""
     lines = []
-    for name in ("cfg1_factors.txt", "cfg2_slice.txt", "cfg3_slice.txt", "many_n_filtered.txt", "OKsort.txt",
+    for name in ("cfg1_factors.txt", "cfg2_windows.txt", "cfg3_windows.txt", "many_n_filtered.txt", "OKsort.txt",
                  "pps_test80t.txt"):
         lines += [l for l in read_lines(name) if l.endswith("+1")]
     out = subprocess.run([orc.REF_FACTOR_PROTH], input="\n".join(lines) + "\n", capture_output=True, text=True,
@@ -331,7 +331,8 @@ def test_negative_fixtures_rejected():
 
 
 def test_goldens_are_true_factors():
-    for name in ("cfg1_factors.txt", "cfg2_slice.txt", "cfg3_slice.txt", "many_n_filtered.txt"):
+    for name in ("cfg1_factors.txt", "cfg2_windows.txt", "cfg3_windows.txt", "cfg2_windows_raw.txt",
+                 "cfg3_windows_raw.txt", "many_n_filtered.txt", "many_n_raw.txt"):
         for l in read_lines(name):
             p, k, n, c = orc.parse_line(l)
             assert (k * pow(2, n, p) + c) % p == 0
@@ -348,3 +349,67 @@ def test_count_primes_equals_listing(p0, p1):
     the listing sieve (itself pinned by sympy above) on count and sum."""
     P = orc.primes(p0, p1)
     assert orc.count_primes(p0, p1, threads=4) == (len(P), int(P.sum(dtype=np.uint64)))
+
+
+def test_no_golden_is_empty():
+    """Every fixture under tests/golden holds data: a golden that is empty
+    would pin a run's factor set vacuously."""
+    names = [n for n in os.listdir(GOLDEN) if n.endswith(".txt")]
+    assert len(names) >= 12
+    for n in names:
+        assert read_lines(n), n
+
+
+def test_count_primes_many_equals_listing():
+    """Several ranges counted together (one table of sieving primes) give
+    each range's listing count and sum; the listing sieve's presieved
+    Miller-Rabin path (narrow windows high up) agrees with sympy."""
+    ranges = [(10**12, 10**12 + 10**6), (3, 10**5), (2**50 + 12345, 2**50 + 3 * 10**6), (2**61, 2**61 + 10**5),
+              (2**62 - 10**5, 2**62)]
+    got = orc.count_primes_many(ranges, threads=4)
+    for (a, b), (c, s) in zip(ranges, got):
+        P = orc.primes(a, b)
+        assert (c, s) == (len(P), int(P.sum(dtype=np.uint64))), (a, b)
+    P = orc.primes(2**61, 2**61 + 3000)
+    assert P.tolist() == list(sympy.primerange(2**61, 2**61 + 3000))
+
+
+def test_check_hits_flags():
+    """orc_check_hits against Python's pow and sympy: golden divisors are
+    valid (and those test_factor keeps are flagged so); a non-divisor, a
+    composite p, k even or out of range, n out of range and a sign the
+    search does not walk are each flagged."""
+    pr = orc.derive(pmin=42070000000000, pmax=42070050000000, kmin=1201, kmax=9999, nmax=2000000, sign=1)
+    raw = [orc.parse_line(l) for l in read_lines("many_n_raw.txt")]
+    kept = {orc.parse_line(l) for l in read_lines("many_n_filtered.txt")}
+    flags = orc.check_hits(pr, _hits(raw), threads=3)
+    for t, f in zip(raw, flags):
+        assert int(f) & ~orc.KEPT == 0, t
+        assert bool(int(f) & orc.KEPT) == (t in kept), t
+    p, k, n, c = raw[0]
+    kr = pow(2, -n, p)  # kr * 2^n - 1 is a multiple of p: a divisor of the -1 form
+    q = int(sympy.nextprime(2**31) * sympy.nextprime(2**31 + 10**6))  # composite, about 2^62
+    kq = (q - pow(2, -n, q)) % q  # kq * 2^n + 1 == 0 (mod q)
+
+    def k_bad(kk):
+        return 0 if kk % 2 and 1201 <= kk <= 9999 else orc.BAD_K
+
+    cases = [((p, k, n + 1, c), orc.BAD_DIVIDES),
+             ((p, k + 1, n, c), orc.BAD_DIVIDES | orc.BAD_K),
+             ((p, k, 2000000, c), orc.BAD_DIVIDES | orc.BAD_N),
+             ((p, k, n, -c), orc.BAD_DIVIDES | orc.BAD_SIGN),
+             ((p, kr, n, -1), orc.BAD_SIGN | k_bad(kr)),
+             ((q, kq, n, 1), orc.BAD_PRIME | k_bad(kq))]
+    got = orc.check_hits(pr, _hits([t for t, _ in cases]))
+    for (t, want), f in zip(cases, got):
+        assert int(f) == want, (t, int(f), want)
+        p_, k_, n_, c_ = t
+        assert ((k_ * pow(2, n_, p_) + c_) % p_ == 0) == (not int(f) & orc.BAD_DIVIDES)
+        assert sympy.isprime(p_) == (not int(f) & orc.BAD_PRIME)
+
+
+def _hits(tuples):
+    a = np.zeros(len(tuples), dtype=orc.HIT_DTYPE)
+    for i, t in enumerate(tuples):
+        a[i] = t
+    return a
