@@ -170,6 +170,20 @@ int orc_is_prime(uint64_t n) {
   return 1;
 }
 
+/* x mod q for q < 2^32 without a 64-bit division (the sieves take one per
+ * sieving prime per segment): a quotient estimate in double precision is
+ * within 2^10 of the true one, the remainder it leaves fits 2^42, and a
+ * second estimate on that is off by at most one. */
+static inline uint64_t mod_u32(uint64_t x, uint32_t q) {
+  const double dq = (double)q;
+  const uint64_t q1 = (uint64_t)((double)x / dq);
+  int64_t r = (int64_t)(x - q1 * (uint64_t)q);
+  r -= (int64_t)((double)r / dq) * (int64_t)q;
+  if (r < 0) r += q;
+  if (r >= (int64_t)q) r -= q;
+  return (uint64_t)r;
+}
+
 /* odd primes <= lim, ascending; returns count (caller frees *out) */
 static uint32_t small_sieve(uint32_t lim, uint32_t **out) {
   uint32_t nodd = lim >= 3 ? (lim - 1) / 2 : 0; /* odd numbers 3..lim */
@@ -218,7 +232,7 @@ static int64_t primes_mr(uint64_t lo, uint64_t p1, uint64_t *out, uint64_t cap, 
       if (q >= 65536 || q * q >= base + 2 * cnt) break;
       uint64_t first = q * q;
       if (first < base) {
-        const uint64_t r = base % q;
+        const uint64_t r = mod_u32(base, (uint32_t)q);
         first = r ? base + (q - r) : base;
         if ((first & 1) == 0) first += q;
       }
@@ -267,7 +281,7 @@ int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
         if (q * q >= base + 2 * cnt) break;
         first = q * q;
         if (first < base) {
-          r = base % q;
+          r = mod_u32(base, (uint32_t)q);
           first = r ? base + (q - r) : base;
           if ((first & 1) == 0) first += q;
         }
@@ -289,7 +303,8 @@ int64_t orc_primes(uint64_t p0, uint64_t p1, uint64_t *out, uint64_t cap) {
 
 /* Count and sum (mod 2^64) of the primes in each of several ranges
  * [p0[i], p1[i]), p0 >= 3, for ranges too large to list: every range is cut
- * into segments of 2^27 odd numbers, and `threads` threads take (range,
+ * into segments of as many odd numbers as there are sieving primes (2^19 to
+ * 2^27), and `threads` threads take (range,
  * segment) jobs in turn, sieving each with every odd prime <= isqrt of the
  * range's end.  The same set as orc_primes (the reference's count= and sum=,
  * main.c:1232).  The odd primes up to the largest isqrt seen are kept for
@@ -347,7 +362,7 @@ static void *count_worker(void *arg) {
       if (q * q >= end) break;
       uint64_t first = q * q;
       if (first < base) {
-        const uint64_t r = base % q;
+        const uint64_t r = mod_u32(base, (uint32_t)q);
         first = r ? base + (q - r) : base;
         if ((first & 1) == 0) first += q;
       }
@@ -388,8 +403,12 @@ int orc_count_primes_many(uint32_t n, const uint64_t *p0, const uint64_t *p1, in
   J.lo = lo;
   J.p1 = p1;
   J.nranges = n;
-  J.seg = UINT64_C(1) << 27;
   J.qs = base_primes((uint32_t)orc_isqrt(top - 1), &J.nq);
+  /* segments small enough to stay in cache while they are marked, large
+   * enough that the one remainder per sieving prime per segment is noise */
+  J.seg = (uint64_t)J.nq;
+  if (J.seg < (UINT64_C(1) << 19)) J.seg = UINT64_C(1) << 19;
+  if (J.seg > (UINT64_C(1) << 27)) J.seg = UINT64_C(1) << 27;
   J.cur = 0;
   J.next = lo[0];
   J.count = count;

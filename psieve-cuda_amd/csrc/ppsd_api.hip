@@ -38,6 +38,17 @@ std::vector<uint32_t> host_primes(uint32_t lim) {
   return out;
 }
 
+// The filter's table: odd primes 11 <= q < 32768 with their reduction
+// constants, padded to whole passes of 64 with copies of the last prime.
+std::vector<SmallPrime> small_prime_table() {
+  std::vector<SmallPrime> sm;
+  const uint64_t two32 = (uint64_t)1 << 32;
+  for (uint32_t q : host_primes(32767))
+    if (q >= 11) sm.push_back({q, (uint32_t)(two32 / q), (uint32_t)(two32 / (q - 1)), (uint32_t)(two32 % q)});
+  while (sm.size() % 64) sm.push_back(sm.back());
+  return sm;
+}
+
 }  // namespace ppsd
 
 static int lg2(uint64_t v) {
@@ -279,13 +290,7 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   OPEN_TRY(hipMalloc(&c->d_blocksum, (c->nbits_max / 32 / kScanBlock + 2) * sizeof(unsigned long long)));
   OPEN_TRY(hipHostMalloc((void **)&c->h_stage, (size_t)c->batch * sizeof(uint64_t), hipHostMallocDefault));
   {
-    std::vector<SmallPrime> sm;
-    for (uint32_t q : host_primes(32767))
-      if (q >= 11) {
-        const uint64_t two32 = (uint64_t)1 << 32;
-        sm.push_back({q, (uint32_t)(two32 / q), (uint32_t)(two32 / (q - 1)), (uint32_t)(two32 % q)});
-      }
-    while (sm.size() % 64) sm.push_back(sm.back());
+    const std::vector<SmallPrime> sm = small_prime_table();
     c->nsmall = (uint32_t)sm.size();
     OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(SmallPrime)));
     OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(SmallPrime), hipMemcpyHostToDevice));

@@ -23,6 +23,7 @@ namespace ppsd {
 int set_err(int code, const char *fmt, ...);
 uint64_t isqrt64(uint64_t x);
 std::vector<uint32_t> host_primes(uint32_t lim);
+std::vector<SmallPrime> small_prime_table();
 
 enum class Pending { none, primes, range };
 

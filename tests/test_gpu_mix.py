@@ -47,22 +47,22 @@ def test_seeded_mix_prefix_batched_vs_oracle():
 
 
 def test_poisson_serving_200_per_s_vs_oracle():
-    """Config-5 arrivals: Poisson at 200 requests/s for 2 s (the mix's
+    """Config-5 arrivals: Poisson at 200 requests/s for 1 s (the mix's
     stream, 3.8x the device's capacity), admitted by the memory budget and
     searched in 50 ms slices while others arrive; after a drain every
     request has completed and each result passes the oracle checks."""
-    rep, rec = ppsd_serve.run(rate=200.0, duration=2.0, drain=240.0, slice_ms=50.0, keep_factors=True)
+    rep, rec = ppsd_serve.run(rate=200.0, duration=1.0, drain=240.0, slice_ms=50.0, keep_factors=True)
     print({k: rep[k] for k in ("requests", "completed", "backlog", "wall_s", "latency_s")}, flush=True)
-    assert rep["failed"] == 0 and rep["backlog"] == 0 and rep["completed"] == rep["requests"] > 300
+    assert rep["failed"] == 0 and rep["backlog"] == 0 and rep["completed"] == rep["requests"] > 150
     rng = np.random.Generator(np.random.PCG64(ppsd_units.MIX_SEED))
     units = [ppsd_units.draw(rng) for _ in range(rep["requests"])]
     params = [u.params for u in units]
     full, checked = mixcheck.check_units(params, [rec[i]["primes"] for i in range(len(params))],
                                          [rec[i]["prime_sum"] for i in range(len(params))],
-                                         [rec[i]["factor_set"] for i in range(len(params))], raw=False, edge=500,
-                                         full_work=5e8)
+                                         [rec[i]["factor_set"] for i in range(len(params))], raw=False, edge=300,
+                                         full_work=2e8)
     print(f"checked {checked} factors; {full} requests walked whole by the oracle", flush=True)
-    assert checked > 1000 and full > 50
+    assert checked > 1000 and full > 30
     # overlap: requests were in flight together (admitted before an earlier one finished)
     order = sorted(rec.values(), key=lambda r: r["admit"])
     assert any(b["admit"] < a["done"] for a, b in zip(order, order[1:]))
