@@ -11,9 +11,11 @@ oracle's independent sieve; --steps K times a prefix of K steps.
 
 Prints one JSON line:  value = primes/s over the timed steps (inputs
 generated on the device, nothing crosses PCIe but the factors),
-roofline = the walk kernel against the INT32 VALU peak, cpu_baseline = the
-CPU oracle (a port of the reference CPU path) on the host's cores over a
-bounded sample of the same workload.
+roofline = the walk kernel's single-rate VALU issue against the chip's
+(CUs x 64 lanes x the measured shader clock), cpu_baseline = the reference's
+own CPU walk (its host program re-walking every prime with test_one_p) on
+the host's cores over a bounded sample of the same workload, with the
+oracle's C port of it beside it.
 """
 import argparse
 import json
@@ -28,17 +30,51 @@ METRIC = "primes/sec sieved (k≤1e4, n≤6e6) batched vs plain; % of INT32 VALU
 
 WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000, sign=1)
 
-# INT32 VALU peak of one MI355X: 256 CUs x 128 lanes x 2.4 GHz = 78.6 TOP/s,
-# i.e. the published 157.3 TFLOP/s FP32 vector peak / 2 flops per FMA.
-PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
-# Algorithmic cost of one window-step (one sign): 4 MUL32 + 11 ALU32 lane-ops
-# (the step's multiplies, carries and reduction, the ctz + shift + compare
-# candidate test), counted as 15 32-bit ops against that peak.
-OPS_PER_STEP = 4 + 11
+# The walk is bound by VALU issue.  Every non-move instruction of its loop
+# (v_mul_lo_u32, v_mad_u64_u32, v_ffbl_b32, v_alignbit_b32, v_min3_u32,
+# v_cmp) is single-rate on gfx950: a wave64 instruction holds its SIMD's 16
+# lanes for 4 clocks, so a CU issues at most 64 single-rate lane-instructions
+# per clock.  The peak is CUs x 64 x the device's peak clock (2400 MHz on
+# MI355X), which bounds the fraction by 1.  The shader clock measured during
+# the launches (s_memtime against s_memrealtime in the walk's first wave) is
+# reported beside it, with the fraction at that clock: the chip runs this
+# load below its peak clock.  The one v_mov_b32 per window retires beside
+# the counted instructions and is not counted.
+LANES_PER_CU_CLOCK = 64
+
+
+def nonmove_per_window(r, u):
+    """Non-move VALU instructions per prime-window of the headline loop (s =
+    32, one sign, odd-low pre-test, three-multiply step), as compiled: per
+    prime-window one v_mul_lo_u32, two v_mad_u64_u32, v_ffbl_b32,
+    v_alignbit_b32 and half a v_min3_u32; per chunk of u windows of r
+    primes one compare (89 per 16 windows at r = 1, u = 16)."""
+    return 5.5 + 1.0 / (r * u)
+
+
+# The 15-op model of one window-step (4 MUL32 + 11 ALU32) against the FP32
+# vector peak's 78.6 T 32-bit ops/s: a ratio of models, reported beside the
+# roofline, not a fraction of anything the hardware does.
+MODEL_OPS_PER_STEP = 4 + 11
+MODEL_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 # HBM bytes one walk launch moves per prime walked, from the FETCH_SIZE and
 # WRITE_SIZE counters (scaled by a 256 MiB copy kernel's readings) on a
 # 2,088,811-prime launch: 16.0 MiB fetched + 4.6 KiB written.
 TRAFFIC_BYTES_PER_PRIME = (16.0 * 2**20 + 4.6 * 2**10) / 2088811
+
+
+def host_cpu():
+    """(threads for CPU work, nproc, CPU model): the process's CPU share --
+    OMP_NUM_THREADS where the machine sets it, else the CPUs this process may
+    run on -- at most the reference's 32 workers (main.h:24)."""
+    share = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return max(1, min(share, 32)), os.cpu_count(), model
 
 
 def parse():
@@ -55,13 +91,52 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(target_s):
-    """The oracle (pps_oracle.c, a C port of test_one_p + test_factor) on the
-    first primes of the workload, full n-range, one thread per host core."""
+def cpu_baseline_reference(threads):
+    """The reference's own CPU walk, timed on the host's cores: its unmodified
+    host (main.c, sieve.c, pps/app.c, pps/factor_proth.c, compiled into
+    oracle/_ref by oracle/Makefile) on libppsd in the appcu layer's flag-all
+    mode, where every n-subsection of every prime is flagged and the host
+    re-walks each prime's whole n-range with its test_one_p
+    (pps/app.c:1037-1121, 1197-1219).  `threads` workers (-t, -d 0) over the
+    first 32768 * threads primes of the workload, batches of 32768 (-m 1).
+    Rate = primes / the program's own sieve time, which covers prime supply
+    and excludes start-up (main.c:1237-1244)."""
+    import math
+    import re
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(HERE, "oracle", "_ref", "ppsieve_ref_on_ppsd")
+    if not os.path.exists(exe):
+        return None
+    lo = WORKLOAD["pmin"]
+    hi = lo + int(32768 * threads * math.log(lo))
+    with tempfile.TemporaryDirectory() as wd:
+        env = dict(os.environ, PPSD_APPCU_FLAG_ALL="1")
+        args = [exe, f"-p{lo}", f"-P{hi}", "-k1201", "-K9999", "-n50000", "-N6000000", "-q", "-d", "0", "-t",
+                str(threads), "-m", "1", "-r", "20"]
+        r = subprocess.run(args, cwd=wd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        print(f"reference CPU baseline failed ({r.returncode}): {r.stderr[-500:]}", file=sys.stderr)
+        return None
+    cnt = re.search(r"count=(\d+),sum=0x([0-9a-f]+)", r.stderr)
+    el = re.search(r"Elapsed time: ([\d.]+) sec\. \(([\d.]+) init \+ ([\d.]+) sieve\)", r.stderr)
+    if not cnt or not el:
+        return None
+    primes, sieve_s = int(cnt.group(1)), float(el.group(3))
+    return {"value": primes / sieve_s, "unit": "primes/s", "cores": threads, "kind": "reference",
+            "sample": f"{primes} primes from 2^50 ([{lo},{hi})), full n-range [50000,6000000), k 1201..9999: "
+                      f"the reference's main.c + pps/app.c test_one_p re-walk of every prime ({threads} workers, "
+                      f"libppsd flag-all), {sieve_s:.1f} s of the program's sieve time"}
+
+
+def cpu_baseline_port(threads, target_s):
+    """The oracle (pps_oracle.c, a C port of the reference's prime supply,
+    test_one_p and test_factor) on the first primes of the workload, full
+    n-range, `threads` threads; the clock covers the prime sieve as well."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pps_oracle as orc
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
     pr = orc.derive(**WORKLOAD)
     P = orc.primes(WORKLOAD["pmin"], WORKLOAD["pmin"] + 40 * 10**6)
     orc.walk(pr, P[:threads], filter=True, threads=threads)  # first call's one-time costs
@@ -70,13 +145,14 @@ def cpu_baseline(target_s):
     orc.walk(pr, P[:threads * pilot], filter=True, threads=threads)
     per = (time.perf_counter() - t0) / pilot  # seconds per prime per thread
     m = int(max(threads * 4, min(len(P), target_s / max(per, 1e-9) * threads)))
-    m = (m // threads) * threads
+    hi = int(P[m - 1]) + 1
     t0 = time.perf_counter()
-    orc.walk(pr, P[:m], filter=True, threads=threads)
+    Q = orc.primes(WORKLOAD["pmin"], hi)
+    orc.walk(pr, Q, filter=True, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": m / dt, "unit": "primes/s", "cores": threads, "kind": "port",
-            "sample": f"first {m} primes >= 2^50 of the workload, full n-range [50000,6000000), "
-                      f"{dt:.1f} s on {threads} threads"}
+    return {"value": len(Q) / dt, "unit": "primes/s", "cores": threads, "kind": "port",
+            "sample": f"the {len(Q)} primes of [2^50, {hi}), full n-range [50000,6000000), sieved and walked "
+                      f"in {dt:.1f} s on {threads} threads"}
 
 
 def mixed_units(n):
@@ -245,7 +321,12 @@ def main():
     value = st.primes / dt
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
     t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
-    achieved = steps_per_launch * OPS_PER_STEP / t_launch / 1e12
+    _, cus = ppsd.device_info(0)
+    clock_hz = st.clock_mhz * 1e6
+    nonmove = nonmove_per_window(shape["r"], shape["u"])
+    achieved = steps_per_launch * nonmove / t_launch / 1e12  # T lane-instructions/s
+    peak = cus * LANES_PER_CU_CLOCK * ppsd.device_clock_mhz(0) * 1e6 / 1e12
+    model_tops = steps_per_launch * MODEL_OPS_PER_STEP / t_launch / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -279,16 +360,28 @@ def main():
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
-            "peak": PEAK_TOPS,
-            "unit": "TOP/s",
-            "frac": achieved / PEAK_TOPS,
+            "peak": peak,
+            "unit": "T single-rate lane-instructions/s",
+            "frac": achieved / peak if peak else None,
             "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
+            "peak_clock_mhz": ppsd.device_clock_mhz(0),
+            "measured_clock_mhz": st.clock_mhz,
+            "frac_at_measured_clock": achieved / (cus * LANES_PER_CU_CLOCK * clock_hz / 1e12) if clock_hz else None,
+            "cus": cus,
+            "nonmove_valu_per_window": nonmove,
+            "model_ratio_15op": model_tops / MODEL_PEAK_TOPS,
         },
     }
     if a.mix_units:
         line["config"]["batched"] = mixed_units(a.mix_units)
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        threads, nproc, model = host_cpu()
+        host = {"nproc": nproc, "cpu_model": model, "threads_used": threads}
+        port = cpu_baseline_port(threads, a.cpu_seconds)
+        ref = cpu_baseline_reference(threads)
+        line["cpu_baseline"] = dict(ref or port, **host)
+        if ref:
+            line["config"]["cpu_port"] = port
     print(json.dumps(line), flush=True)
 
 

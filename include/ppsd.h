@@ -65,6 +65,8 @@ typedef struct ppsd_stats {
   double sieve_ms;        /* on-device prime supply kernels */
   double filter_ms;       /* small-prime filter kernel */
   uint64_t retries;       /* batches re-walked in halves after a full hit ring */
+  double clock_mhz;       /* shader clock during the last walk launch (its first wave's s_memtime
+                             cycles over s_memrealtime's 100 MHz ticks); 0 before any */
 } ppsd_stats;
 
 typedef struct ppsd_ctx ppsd_ctx;
@@ -77,6 +79,9 @@ int ppsd_derive_params(ppsd_params *p);
 
 /* Number of HIP devices (0 without a GPU). */
 int ppsd_device_count(void);
+
+/* Peak shader clock of a device in MHz (its reported clock rate). */
+int ppsd_device_clock(int device, uint32_t *max_mhz);
 
 /* Architecture name (e.g. "gfx950:sramecc+:xnack-") and compute-unit count
  * of a device; name may be NULL. */

@@ -78,6 +78,16 @@ bool twin_mode() {
   return e && *e && *e != '0';
 }
 
+// Measurement mode, PPSD_APPCU_FLAG_ALL=1: get_factors_found flags every
+// n-subsection of every prime, so that the host re-walks the whole n-range
+// of each prime on the CPU with its own test_one_p (pps/app.c:1037-1121,
+// 1197-1219).  That times the reference's CPU walk through its unmodified
+// code; the results are unchanged (a re-walk finds exactly the divisors).
+bool flag_all_mode() {
+  const char *e = getenv("PPSD_APPCU_FLAG_ALL");
+  return e && *e && *e != '0';
+}
+
 void device_error(const char *where) {
   fprintf(stderr, "Cuda error: %s: %s\n", where, ppsd_last_error());  // checkCUDAErr, appcu.cu:124-131
   cuda_finalize();
@@ -198,6 +208,12 @@ void get_factors_found(unsigned char *factor_found, const unsigned int cthread_c
   if (ppsd_collect_raw(w.ctx, nullptr, 0, &n, nullptr, nullptr) != 0) device_error("get_factors_found");
   w.hits.resize(n);
   if (n && ppsd_collect_raw(w.ctx, w.hits.data(), n, &n, nullptr, nullptr) != 0) device_error("get_factors_found");
+  if (flag_all_mode()) {
+    // every prime of the batch; a flushed batch is padded with multiples of
+    // 3 (pps/app.c:1261-1266), which are left out
+    for (unsigned int i = 0; i < cthread_count && i < w.P.size(); ++i) factor_found[i] = w.P[i] % 3 ? 0xff : 0;
+    return;
+  }
   const unsigned int *start = thread_subsections[th].n_subsection_start;
   for (const ppsd_factor &h : w.hits) {
     int j = 0;

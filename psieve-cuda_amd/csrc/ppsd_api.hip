@@ -121,6 +121,14 @@ int ppsd_device_count(void) {
 
 const char *ppsd_last_error(void) { return g_err.c_str(); }
 
+int ppsd_device_clock(int device, uint32_t *max_mhz) {
+  int khz = 0;
+  const hipError_t e = hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, device);
+  if (e != hipSuccess) return set_err(PPSD_ERR_NOT_IMPLEMENTED, "GPU %d: %s", device, hipGetErrorString(e));
+  if (max_mhz) *max_mhz = (uint32_t)(khz / 1000);
+  return 0;
+}
+
 int ppsd_device_info(int device, char *name, uint32_t name_len, uint32_t *cus) {
   hipDeviceProp_t prop;
   const hipError_t e = hipGetDeviceProperties(&prop, device);
@@ -151,6 +159,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_P);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_counters);
+  (void)hipFree(c->d_clk);
   (void)hipFree(c->d_hits);
   (void)hipFree(c->d_kept);
   (void)hipFree(c->d_small);
@@ -283,6 +292,7 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   OPEN_TRY(hipMalloc(&c->d_P, (size_t)c->batch * sizeof(uint64_t)));
   OPEN_TRY(hipMalloc(&c->d_count, sizeof(uint32_t)));
   OPEN_TRY(hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)));
+  OPEN_TRY(hipMalloc(&c->d_clk, 4 * sizeof(unsigned long long)));
   OPEN_TRY(hipMalloc(&c->d_hits, c->hit_cap * sizeof(Hit)));
   OPEN_TRY(hipMalloc(&c->d_kept, c->hit_cap * sizeof(Hit)));
   OPEN_TRY(hipMalloc(&c->d_bitmap, (c->nbits_max / 32 + 1) * sizeof(uint32_t)));
@@ -411,6 +421,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   a.hits = c->d_hits;
   a.hit_count = c->d_counters;
   a.hit_cap = c->hit_cap;
+  a.clk = c->d_clk;
   const uint32_t per_block = (uint32_t)(c->tune_wg * c->tune_r);
   const uint32_t grid = (grid_primes + per_block - 1) / per_block;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
@@ -470,25 +481,36 @@ int ppsd_submit_primes(ppsd_ctx *c, const uint64_t *P, uint32_t n) {
   return 0;
 }
 
-ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u) {
+ppsd::WalkForm ppsd::choose_form(const ppsd_params &p, int r, int u, int mode, bool plain) {
   const bool oddlo_ok = (p.kmax >> 32) == 0;
   if (mode == 1) return {PRE_EXACT, u};
   if (mode == 2) return {PRE_RANGE, u};
   if (mode == 3) return {oddlo_ok ? PRE_ODDLO : PRE_EXACT, u};
   // false alarms per lane-window (the smallest p of the unit fires most)
-  const double signs = p.sign == 0 ? 2.0 : 1.0;
+  const int both = p.sign == 0 ? 1 : 0;
+  const double signs = both ? 2.0 : 1.0;
   const double f_range = signs * (double)p.kmax * ldexp(1.0, (int)p.nstep - 1) / (double)p.pmin;
   const double f_odd = signs * ldexp((double)p.kmax + 1.0, -32);
-  const double base_range = p.sign == 0 ? 0.47 : 0.69;
-  const double base_odd = p.sign == 0 ? 0.89 : 0.92;
+  // Time per window relative to the exact form's at u = 16, measured on
+  // MI355X (r = 1, p from 2^50 to 2^61), at u = 4 / 8 / 16 and negligible
+  // false alarms: the range form 0.70 / 0.66 / 0.64 (one sign) and 0.455 /
+  // 0.419 / 0.401 (both signs); the odd-low form 0.98 / 0.94 / 0.90 and
+  // 0.94 / 0.90 / 0.86.
+  static const double base_range[2][3] = {{0.70, 0.66, 0.64}, {0.455, 0.419, 0.401}};
+  static const double base_odd[2][3] = {{0.98, 0.94, 0.90}, {0.94, 0.90, 0.86}};
   WalkForm best{PRE_EXACT, u};
   double best_cost = 1.0;
-  for (int cu = u, halvings = 0; cu >= 4 && (free_u || cu == u); cu /= 2, ++halvings) {
-    // the range form must win by a margin of 0.03, the odd-low form by 0.01;
-    // a range-form false alarm re-walks its chunk with the range compare
-    // (about half an exact chunk), an odd-low one with the exact pre-test
-    const double range = base_range + 0.5 * f_range * 64.0 * r * cu + 0.02 * halvings + 0.03;
-    const double odd = base_odd + f_odd * 64.0 * r * cu + 0.02 * halvings + 0.01;
+  for (int cu = u; cu >= 4 && (plain || cu == u); cu /= 2) {
+    const int lvl = cu >= 16 ? 2 : cu >= 8 ? 1 : 0;
+    // A false alarm costs, in exact window-times: a range chunk of the plain
+    // walk is re-checked from its kept states, about 2.0 + 0.1 u; other
+    // chunks are walked again, about u/2 with the range compare and u with
+    // the exact pre-test that odd-low alarms use.  A wave's chunk holds
+    // 64 r u states, so the cost per window is 64 r f times that over u.
+    // The range form must win by a margin of 0.03, the odd-low form by 0.01.
+    const double range_alarm = plain ? 2.0 + 0.1 * cu : 0.5 * cu;
+    const double range = base_range[both][lvl] + f_range * 64.0 * r * range_alarm + 0.03;
+    const double odd = base_odd[both][lvl] + f_odd * 64.0 * r * cu + 0.01;
     if (range < best_cost) {
       best_cost = range;
       best = {PRE_RANGE, cu};
@@ -568,6 +590,12 @@ int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
   return 0;
 }
 
+// The shader clock of the last walk launch, from its clock samples: cycles
+// of s_memtime over ticks of the 100 MHz s_memrealtime.
+static void note_clock(ppsd_ctx *c, const unsigned long long clk[4]) {
+  if (clk[3] > clk[1] && clk[2] > clk[0]) c->st.clock_mhz = (double)(clk[2] - clk[0]) / (double)(clk[3] - clk[1]) * 100.0;
+}
+
 float ppsd::elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
@@ -582,12 +610,14 @@ static int walk_pieces(ppsd_ctx *c, bool raw, uint32_t first, uint32_t n, uint64
     auto [f, k] = todo.back();
     todo.pop_back();
     if (k == 0) continue;
-    unsigned long long cnt[2];
+    unsigned long long cnt[2], clk[4] = {0, 0, 0, 0};
     HIP_TRY(launch_zero_u64(c->d_counters, 2, c->stream));
     int rc = enqueue_walk(c, f, k, nullptr, k);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(clk, c->d_clk, sizeof clk, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    note_clock(c, clk);
     c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
     c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
     if (cnt[0] > c->hit_cap) {
@@ -652,11 +682,13 @@ static int finish(ppsd_ctx *c, bool raw) {
     return set_err(PPSD_ERR_STATE, "nothing submitted");
   }
   HIP_TRY(hipSetDevice(c->device));
-  unsigned long long cnt[4];
+  unsigned long long cnt[4], clk[4] = {0, 0, 0, 0};
   uint32_t nprimes = c->n_primes_host;
   HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(clk, c->d_clk, sizeof clk, hipMemcpyDeviceToHost, c->stream));
   if (c->pending == Pending::range) HIP_TRY(hipMemcpyAsync(&nprimes, c->d_count, sizeof nprimes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  note_clock(c, clk);
   c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
   c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
   if (c->sieved) c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);

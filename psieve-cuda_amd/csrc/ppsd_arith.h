@@ -117,27 +117,28 @@ __device__ __forceinline__ uint64_t step(uint64_t K, uint64_t p, uint32_t ps_sh,
 
 // The window step in three multiplies, for primes with
 // ((p-1) >> s) + lo32(p) < 2^32 (at s = 32: lo32(p) + hi32(p) < 2^32).
-// The ceiling term and X = K >> s go into one addend: the 64-bit value
-// S = m*p_lo + (X*2^32 + 2^32 - 1) has the high word X + ceil(m*p_lo / 2^32),
-// so K' = m*p_hi + hi32(S).  That needs X < 2^32 and S < 2^64: the ceiling
-// is at most p_lo (m < 2^32) and X at most (p-1) >> s (K < p), so the bound
-// above gives both.  X is hi32(K) at s = 32 and v_alignbit_b32(hi32(K),
-// lo32(K), s) -- the low word of K >> s, all of it here -- below.  The step
-// is then one v_mul_lo_u32 and two v_mad_u64_u32 (plus v_alignbit_b32 at
-// s < 32, in place of the 64-bit shift of step()); the moves that place the
-// addends' words (2^32 - 1 below X, hi32(S) below the zero register) are
-// v_mov_b32.  Primes that fail the bound have lo32(p) within (p-1) >> s of
-// 2^32, i.e. they lie at the top of each 2^32 block, so in an ascending
-// batch they form runs and whole waves take one step or the other.
+// With L = K << (32-s) mod 2^64 -- the 64-bit value X*2^32 + (lo_s(K) <<
+// (32-s)) for X = K >> s, which needs X < 2^32 -- the sum
+// S = m*p_lo + L has a zero low word (lo32(m*p_lo) == -(lo_s(K) << (32-s))
+// mod 2^32) and the high word X + hi32(m*p_lo) + [lo_s(K) != 0], which is
+// X + ceil(m*p_lo / 2^32); so K' = m*p_hi + hi32(S).  S < 2^64 because
+// m*p_lo <= (2^32-1)*p_lo and L < (X+1)*2^32 with X <= (p-1) >> s, which the
+// bound above keeps below 2^64 in sum.  At s = 32, L is K itself: the first
+// mad takes the state's own register pair as its addend, with no move.  The
+// step is then one v_mul_lo_u32 and two v_mad_u64_u32 (plus v_lshlrev_b64 at
+// s < 32, where step() needs a 64-bit shift too), and one v_mov_b32 that
+// places hi32(S) beside the zero register for the second mad.  Primes that
+// fail the bound have lo32(p) within (p-1) >> s of 2^32, i.e. they lie at
+// the top of each 2^32 block, so in an ascending batch they form runs and
+// whole waves take one step or the other.
 __device__ __forceinline__ bool fused_step_ok(uint64_t p, uint32_t s) {
   return ((p - 1) >> s) + lo32(p) <= 0xffffffffull;
 }
 template <bool S32>
 __device__ __forceinline__ uint64_t step_fused(uint64_t K, uint64_t p, uint32_t ps_sh, uint32_t s, uint32_t zero) {
   const uint32_t m = lo32(K) * ps_sh;
-  const uint32_t x = S32 ? hi32(K) : __builtin_amdgcn_alignbit(hi32(K), lo32(K), s);
-  const uint64_t a = (uint64_t)x << 32 | 0xffffffffull;
-  const uint32_t hs = hi32(mad_u64_u32(m, lo32(p), a));
+  const uint64_t L = S32 ? K : K << (32u - s);
+  const uint32_t hs = hi32(mad_u64_u32(m, lo32(p), L));
   return mad_u64_u32(m, hi32(p), (uint64_t)zero << 32 | hs);
 }
 

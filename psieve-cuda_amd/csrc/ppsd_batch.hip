@@ -58,7 +58,9 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P,
   const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s, P.kmax << (P.s - 1)};
   start_states<SIGN, R>(L, S, n0, P.s, rule.bound);
   const PieceSink sink{a.hits, a.hit_count, a.hit_cap, T.piece};
-  walk_windows<SIGN, R, U, S32, PRE>(L, S, n0, T.nw, rule, sink);
+  // states not kept: the persistent kernel holds every form, and its
+  // registers (so its waves per SIMD) are set by the largest
+  walk_windows<SIGN, R, U, S32, PRE, false>(L, S, n0, T.nw, rule, sink);
 }
 
 template <int R, int U, int PRE>

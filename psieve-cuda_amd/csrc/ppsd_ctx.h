@@ -83,6 +83,7 @@ struct ppsd_ctx {
   uint64_t *d_P = nullptr;
   uint32_t *d_count = nullptr;             // primes in d_P (range mode)
   unsigned long long *d_counters = nullptr;  // [0] raw hits, [1] kept, [2] prime sum
+  unsigned long long *d_clk = nullptr;       // clock samples of the last walk launch (WalkArgs::clk)
   Hit *d_hits = nullptr;
   Hit *d_kept = nullptr;
   uint64_t hit_cap = 0;        // capacity in use
@@ -166,19 +167,15 @@ double hits_per_prime(const ppsd_params &p, uint64_t lo);
 // How a unit is walked at kernel shape (r, u): the pre-test form (PRE_EXACT,
 // PRE_RANGE or PRE_ODDLO) and the chunk length.  mode 1, 2 and 3 force a
 // form (3 falls back to PRE_EXACT when kmax >= 2^32); mode 0 prices them
-// with costs measured on MI355X, relative to the exact form's time per
-// window: the range form costs 0.52 (both signs) or 0.75 (one sign), the
-// odd-low form 0.89 or 0.92, each plus 1.0 per expected false alarm of a
-// wave's chunk (a re-walk of the chunk with the exact pre-test; 64*r*u times
-// the alarm rate per state: bound/p for the range form, kmax/2^32 for the
-// other; twice that for both signs) and 0.02 for each halving of the chunk
-// below u.  Where chunks are shared by many units
-// (batched pieces) the chunk stays u (free_u false).
+// with the costs measured on MI355X that choose_form lists.  plain: the
+// plain walk kernel, which may shorten the chunk below u and keeps a range
+// chunk's states for its re-check; otherwise the persistent kernel, whose
+// chunk stays u (it is shared by many units) and which re-walks.
 struct WalkForm {
   int pre;
   int u;
 };
-WalkForm choose_form(const ppsd_params &p, int r, int u, int mode, bool free_u);
+WalkForm choose_form(const ppsd_params &p, int r, int u, int mode, bool plain);
 // Milliseconds between two recorded events (0 on error).
 float elapsed(hipEvent_t a, hipEvent_t b);
 // Batched walk: buffers (allocated on first use), sieving primes for every

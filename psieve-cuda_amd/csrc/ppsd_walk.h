@@ -111,14 +111,10 @@ struct LaneStates {
   uint64_t K[R];
 };
 
-// Re-walk `cnt` windows from state K (window start n).  Reached only when a
-// pre-test fired somewhere in the chunk: a false alarm of a cheap form, or a
-// hit.  Each window first gets a per-window pre-test, RPRE: the exact one
-// (three instructions per state), or for the range form its 64-bit compare
-// (one per sign, which is all the alarm needs to be located); only a window
-// that passes it gets the full test and the emission, so a false alarm
-// costs little more than walking the chunk again.  FUSED: the wave's step
-// (step_fused), as in test_and_step.
+// Re-walk `cnt` windows from state K (window start n), for chunks whose
+// states were not kept: the same per-window pre-test, full test and
+// emission as recheck, stepping again between windows.  FUSED: the wave's
+// step (step_fused), as in test_and_step.
 template <int SIGN, int R, bool S32, bool FUSED, int RPRE, class Sink>
 __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, uint32_t n, uint32_t cnt,
                                        const WindowRule &a, const Sink &sink) {
@@ -150,11 +146,48 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
   }
 }
 
-// Pre-test of W windows of R states each, stepping them: true when one of
-// them may hold a hit.  FUSED: the three-multiply step, for a wave whose
-// primes all pass fused_step_ok.
+// Re-check the W windows of a chunk whose pre-test fired, from the states
+// the chunk passed through (kept in registers while it was walked, so no
+// step is repeated).  Reached only when a pre-test fired somewhere in the
+// chunk: a false alarm of a cheap form, or a hit.  Each window first gets a
+// per-window pre-test, RPRE: the exact one (three instructions per state),
+// or for the range form its 64-bit compare (one per sign, which is all the
+// alarm needs to be located); only a window that passes it (wave-wide) gets
+// the full test and the emission.
+template <int SIGN, int R, int W, int RPRE, class Sink>
+__device__ __forceinline__ void recheck(const LanePrimes<R> &L, const LaneStates<R> (&hist)[W], uint32_t n,
+                                        const WindowRule &a, const Sink &sink) {
+#pragma unroll
+  for (int u = 0; u < W; ++u, n += a.s) {
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) any |= pre<SIGN, RPRE>(hist[u].K[r], L.p[r], L.pmb[r], a);
+    if (__ballot(any)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t Kc = hist[u].K[r];
+        uint64_t k = 0;
+        uint32_t nn = 0;
+        if (SIGN == 0) {
+          bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
+          emit(h, L.p[r], k, nn, -1, sink);
+          h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
+          emit(h, L.p[r], k, nn, +1, sink);
+        } else {
+          const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
+          emit(h, L.p[r], k, nn, SIGN, sink);
+        }
+      }
+    }
+  }
+}
+
+// Pre-test of W windows of R states each, stepping them and keeping each
+// window's states in hist: true when one of them may hold a hit.  FUSED:
+// the three-multiply step, for a wave whose primes all pass fused_step_ok.
 template <int SIGN, int R, int W, bool S32, int PRE, bool FUSED>
-__device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a) {
+__device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates<R> &S, const WindowRule &a,
+                                              LaneStates<R> (&hist)[W]) {
   bool any = false;
   uint32_t lowest = ~0u;
   uint32_t highest[R];
@@ -162,6 +195,7 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
   for (int r = 0; r < R; ++r) highest[r] = 0;
 #pragma unroll
   for (int u = 0; u < W; ++u) {
+    hist[u] = S;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (PRE == PRE_ODDLO) {
@@ -191,20 +225,30 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
   return any;
 }
 
-template <int SIGN, int R, int U, bool S32, int PRE, bool FUSED, class Sink>
+// A chunk whose pre-test fired is checked window by window: with KEEP from
+// the states it passed through (U*R more state registers, no re-stepping),
+// else by re-walking it from a snapshot.  KEEP pays where false alarms are
+// frequent (the range form, on up to a tenth of the chunks).
+template <int SIGN, int R, int U, bool S32, int PRE, bool FUSED, bool KEEP, class Sink>
 __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                             const WindowRule &a, const Sink &sink) {
   constexpr int RPRE = PRE == PRE_RANGE ? PRE_RANGE : PRE_EXACT;
   const uint32_t s = a.s;
   uint32_t w = 0;
   for (; w + U <= nwin; w += U) {
+    LaneStates<R> hist[U];
     const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, U, a, sink);
+    if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist)) {
+      if (KEEP)
+        recheck<SIGN, R, U, RPRE>(L, hist, n, a, sink);
+      else
+        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, U, a, sink);
+    }
     n += U * s;
   }
   for (; w < nwin; ++w, n += s) {
-    const LaneStates<R> snap = S;
-    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a)) rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, 1, a, sink);
+    LaneStates<R> hist[1];
+    if (test_and_step<SIGN, R, 1, S32, PRE, FUSED>(L, S, a, hist)) recheck<SIGN, R, 1, RPRE>(L, hist, n, a, sink);
   }
 }
 
@@ -213,16 +257,16 @@ __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R
 // (then it is walked again with exact tests).  A wave whose primes all pass
 // fused_step_ok walks with the three-multiply step (a wave-uniform choice,
 // so no lane diverges).
-template <int SIGN, int R, int U, bool S32, int PRE, class Sink>
+template <int SIGN, int R, int U, bool S32, int PRE, bool KEEP, class Sink>
 __device__ __forceinline__ void walk_windows(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                              const WindowRule &a, const Sink &sink) {
   bool ok = true;
 #pragma unroll
   for (int r = 0; r < R; ++r) ok = ok && fused_step_ok(L.p[r], a.s);
   if (__all(ok))
-    walk_chunks<SIGN, R, U, S32, PRE, true>(L, S, n, nwin, a, sink);
+    walk_chunks<SIGN, R, U, S32, PRE, true, KEEP>(L, S, n, nwin, a, sink);
   else
-    walk_chunks<SIGN, R, U, S32, PRE, false>(L, S, n, nwin, a, sink);
+    walk_chunks<SIGN, R, U, S32, PRE, false, KEEP>(L, S, n, nwin, a, sink);
 }
 
 // Start states: K = -c * 2^-e mod p (both signs: 2^-e, whose negation p - K

@@ -47,7 +47,7 @@ class Stats(C.Structure):
         ("window_steps", C.c_uint64), ("raw_hits", C.c_uint64),
         ("factors", C.c_uint64), ("walk_launches", C.c_uint64),
         ("walk_ms", C.c_double), ("sieve_ms", C.c_double), ("filter_ms", C.c_double),
-        ("retries", C.c_uint64),
+        ("retries", C.c_uint64), ("clock_mhz", C.c_double),
     ]
 
 
@@ -81,6 +81,7 @@ def _load():
     sig = {
         "ppsd_derive_params": (C.c_int, [_P(Params)]),
         "ppsd_device_count": (C.c_int, []),
+        "ppsd_device_clock": (C.c_int, [C.c_int, _P(C.c_uint32)]),
         "ppsd_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_uint32, _P(C.c_uint32)]),
         "ppsd_open": (C.c_int, [C.c_int, _P(Params), C.c_uint32, _P(C.c_void_p)]),
         "ppsd_batch_size": (C.c_uint32, [C.c_void_p]),
@@ -188,6 +189,13 @@ def device_info(device: int = 0):
     cus = C.c_uint32()
     _check(lib().ppsd_device_info(device, buf, len(buf), C.byref(cus)), "ppsd_device_info")
     return buf.value.decode(), cus.value
+
+
+def device_clock_mhz(device: int = 0) -> int:
+    """Peak shader clock of a device (its reported clock rate), MHz."""
+    mhz = C.c_uint32()
+    _check(lib().ppsd_device_clock(device, C.byref(mhz)), "ppsd_device_clock")
+    return mhz.value
 
 
 def _ptr(a: np.ndarray):

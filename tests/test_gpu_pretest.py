@@ -79,13 +79,14 @@ def test_dense_units_each_form(seed):
 
 
 def test_automatic_choice():
-    """Automatic: the range form, on a chunk of 4 windows, where
-    kmax*2^(s-1)/p is tiny (2^61, both signs, kmax 1e5: 0.05 false alarms
-    per chunk of a wave); the odd-low form where kmax/2^32 is (2^50 and
+    """Automatic: the range form where kmax*2^(s-1)/p is tiny (2^61, both
+    signs, kmax 1e5: 0.1 false alarms per chunk of 8 windows of a wave,
+    each re-checked from the chunk's kept states); the odd-low form where
+    kmax/2^32 is (2^50 and
     4.2e13 with kmax 9999: 1.2e-3 false alarms per chunk); the exact form
     where neither is (2^46, kmax 1e6); a forced form keeps the tuned chunk."""
     big_k = dict(pmin=2**46, pmax=2**46 + 10**6, kmin=3, kmax=999999, nmin=100000, nmax=2000000)
-    for tag, kw, want in (("cfg3", RUNS["cfg3_windows"]["params"], (0, RANGE, 4)),
+    for tag, kw, want in (("cfg3", RUNS["cfg3_windows"]["params"], (0, RANGE, 8)),
                           ("cfg2", RUNS["cfg2_windows"]["params"], (0, ODDLO, 8)),
                           ("many_n", RUNS["many_n"]["params"], (0, ODDLO, 8)),
                           ("2^46 kmax 1e6", big_k, (0, EXACT, 8))):
