@@ -263,6 +263,31 @@ int ppsd_serve_get_stats(ppsd_server *s, ppsd_serve_stats *st);
 /* Stops the worker after its current slice; unfinished requests are dropped. */
 int ppsd_serve_close(ppsd_server *s);
 
+/* ---- Allocation-free walk.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It is the
+ * one-call, context-free form of ppsd_submit_primes followed by
+ * ppsd_collect (raw == 0) or ppsd_collect_raw (raw != 0), on primes and an
+ * output buffer that are already in device memory, and it allocates
+ * nothing: its raw-hit ring (PPSD_NOALLOC_RING_HITS records), counters and
+ * the filter's small-prime table live in a fixed device array of the
+ * library (about 12.6 MB per GPU, held until the process ends).
+ *
+ * P: n device-resident primes, odd in [pmin, 2^62), any order, n <=
+ * PPSD_NOALLOC_MAX_PRIMES (more is PPSD_ERR_INVALID_PARAM).  out: a device
+ * buffer of cap ppsd_factor records; the first min(*n_out, cap) results are
+ * written there in no particular order, *n_out receives their number.
+ * stream: a hipStream_t to run on (NULL: the device's null stream); the call
+ * returns when the work is done.  p is used as ppsd_open takes it, with the
+ * library's default kernel shape.  Every call on one GPU uses that GPU's one
+ * array: calls that overlap on one GPU race and must be serialised by the
+ * caller.  A prime with more than PPSD_NOALLOC_RING_HITS raw divisors is
+ * PPSD_ERR_INSUFFICIENT_RESOURCE. */
+#define PPSD_NOALLOC_RING_HITS (1u << 19)
+#define PPSD_NOALLOC_MAX_PRIMES (1u << 24)
+int ppsd_walk_primes_noalloc(int device, const ppsd_params *p, const uint64_t *P, uint32_t n, ppsd_factor *out,
+                             uint64_t cap, int raw, void *stream, uint64_t *n_out);
+
 /* Arithmetic check (no counterpart in the reference): out[i] = 2^-e mod P[i]
  * after nwin further windows of s, i.e. 2^-(e + nwin*s) mod P[i], computed by
  * the same device code as the walk, returned as the walk holds it (the

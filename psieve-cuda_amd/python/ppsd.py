@@ -114,6 +114,8 @@ def _load():
         "ppsd_serve_release": (C.c_int, [C.c_void_p, C.c_uint64]),
         "ppsd_serve_get_stats": (C.c_int, [C.c_void_p, _P(ServeStats)]),
         "ppsd_serve_close": (C.c_int, [C.c_void_p]),
+        "ppsd_walk_primes_noalloc": (C.c_int, [C.c_int, _P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                               C.c_int, C.c_void_p, _u64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
