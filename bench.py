@@ -12,7 +12,7 @@ oracle's independent sieve; --steps K times a prefix of K steps.
 Prints one JSON line:  value = primes/s over the timed steps (inputs
 generated on the device, nothing crosses PCIe but the factors),
 roofline = the walk kernel's single-rate VALU issue against the chip's
-(CUs x 64 lanes x the measured shader clock), cpu_baseline = the reference's
+(CUs x 64 lanes x the peak shader clock), cpu_baseline = the reference's
 own CPU walk (its host program re-walking every prime with test_one_p) on
 the host's cores over a bounded sample of the same workload, with the
 oracle's C port of it beside it.
@@ -35,11 +35,10 @@ WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=
 # v_cmp) is single-rate on gfx950: a wave64 instruction holds its SIMD's 16
 # lanes for 4 clocks, so a CU issues at most 64 single-rate lane-instructions
 # per clock.  The peak is CUs x 64 x the device's peak clock (2400 MHz on
-# MI355X), which bounds the fraction by 1.  The shader clock measured during
-# the launches (s_memtime against s_memrealtime in the walk's first wave) is
-# reported beside it, with the fraction at that clock: the chip runs this
-# load below its peak clock.  The one v_mov_b32 per window retires beside
-# the counted instructions and is not counted.
+# MI355X), which bounds the fraction by 1; under this load the chip runs
+# below its peak clock, which the counter profile's cycle counts show.  The
+# one v_mov_b32 per window retires beside the counted instructions and is
+# not counted.
 LANES_PER_CU_CLOCK = 64
 
 
@@ -322,7 +321,6 @@ def main():
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
     t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
     _, cus = ppsd.device_info(0)
-    clock_hz = st.clock_mhz * 1e6
     nonmove = nonmove_per_window(shape["r"], shape["u"])
     achieved = steps_per_launch * nonmove / t_launch / 1e12  # T lane-instructions/s
     peak = cus * LANES_PER_CU_CLOCK * ppsd.device_clock_mhz(0) * 1e6 / 1e12
@@ -365,8 +363,6 @@ def main():
             "frac": achieved / peak if peak else None,
             "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
             "peak_clock_mhz": ppsd.device_clock_mhz(0),
-            "measured_clock_mhz": st.clock_mhz,
-            "frac_at_measured_clock": achieved / (cus * LANES_PER_CU_CLOCK * clock_hz / 1e12) if clock_hz else None,
             "cus": cus,
             "nonmove_valu_per_window": nonmove,
             "model_ratio_15op": model_tops / MODEL_PEAK_TOPS,

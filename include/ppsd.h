@@ -65,8 +65,6 @@ typedef struct ppsd_stats {
   double sieve_ms;        /* on-device prime supply kernels */
   double filter_ms;       /* small-prime filter kernel */
   uint64_t retries;       /* batches re-walked in halves after a full hit ring */
-  double clock_mhz;       /* shader clock during the last walk launch (its first wave's s_memtime
-                             cycles over s_memrealtime's 100 MHz ticks); 0 before any */
 } ppsd_stats;
 
 typedef struct ppsd_ctx ppsd_ctx;

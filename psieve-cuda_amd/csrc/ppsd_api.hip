@@ -159,7 +159,6 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_P);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_counters);
-  (void)hipFree(c->d_clk);
   (void)hipFree(c->d_hits);
   (void)hipFree(c->d_kept);
   (void)hipFree(c->d_small);
@@ -292,7 +291,6 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
   OPEN_TRY(hipMalloc(&c->d_P, (size_t)c->batch * sizeof(uint64_t)));
   OPEN_TRY(hipMalloc(&c->d_count, sizeof(uint32_t)));
   OPEN_TRY(hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)));
-  OPEN_TRY(hipMalloc(&c->d_clk, 4 * sizeof(unsigned long long)));
   OPEN_TRY(hipMalloc(&c->d_hits, c->hit_cap * sizeof(Hit)));
   OPEN_TRY(hipMalloc(&c->d_kept, c->hit_cap * sizeof(Hit)));
   OPEN_TRY(hipMalloc(&c->d_bitmap, (c->nbits_max / 32 + 1) * sizeof(uint32_t)));
@@ -421,7 +419,6 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   a.hits = c->d_hits;
   a.hit_count = c->d_counters;
   a.hit_cap = c->hit_cap;
-  a.clk = c->d_clk;
   const uint32_t per_block = (uint32_t)(c->tune_wg * c->tune_r);
   const uint32_t grid = (grid_primes + per_block - 1) / per_block;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
@@ -590,12 +587,6 @@ int ppsd_submit_range(ppsd_ctx *c, uint64_t p0, uint64_t p1, uint64_t *p_next) {
   return 0;
 }
 
-// The shader clock of the last walk launch, from its clock samples: cycles
-// of s_memtime over ticks of the 100 MHz s_memrealtime.
-static void note_clock(ppsd_ctx *c, const unsigned long long clk[4]) {
-  if (clk[3] > clk[1] && clk[2] > clk[0]) c->st.clock_mhz = (double)(clk[2] - clk[0]) / (double)(clk[3] - clk[1]) * 100.0;
-}
-
 float ppsd::elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
@@ -610,14 +601,12 @@ static int walk_pieces(ppsd_ctx *c, bool raw, uint32_t first, uint32_t n, uint64
     auto [f, k] = todo.back();
     todo.pop_back();
     if (k == 0) continue;
-    unsigned long long cnt[2], clk[4] = {0, 0, 0, 0};
+    unsigned long long cnt[2];
     HIP_TRY(launch_zero_u64(c->d_counters, 2, c->stream));
     int rc = enqueue_walk(c, f, k, nullptr, k);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(clk, c->d_clk, sizeof clk, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    note_clock(c, clk);
     c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
     c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
     if (cnt[0] > c->hit_cap) {
@@ -682,13 +671,11 @@ static int finish(ppsd_ctx *c, bool raw) {
     return set_err(PPSD_ERR_STATE, "nothing submitted");
   }
   HIP_TRY(hipSetDevice(c->device));
-  unsigned long long cnt[4], clk[4] = {0, 0, 0, 0};
+  unsigned long long cnt[4];
   uint32_t nprimes = c->n_primes_host;
   HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(clk, c->d_clk, sizeof clk, hipMemcpyDeviceToHost, c->stream));
   if (c->pending == Pending::range) HIP_TRY(hipMemcpyAsync(&nprimes, c->d_count, sizeof nprimes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  note_clock(c, clk);
   c->st.walk_ms += elapsed(c->ev[0], c->ev[1]);
   c->st.filter_ms += elapsed(c->ev[4], c->ev[5]);
   if (c->sieved) c->st.sieve_ms += elapsed(c->ev[2], c->ev[3]);

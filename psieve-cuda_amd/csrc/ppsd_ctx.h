@@ -83,7 +83,6 @@ struct ppsd_ctx {
   uint64_t *d_P = nullptr;
   uint32_t *d_count = nullptr;             // primes in d_P (range mode)
   unsigned long long *d_counters = nullptr;  // [0] raw hits, [1] kept, [2] prime sum
-  unsigned long long *d_clk = nullptr;       // clock samples of the last walk launch (WalkArgs::clk)
   Hit *d_hits = nullptr;
   Hit *d_kept = nullptr;
   uint64_t hit_cap = 0;        // capacity in use

@@ -38,10 +38,6 @@ struct WalkArgs {
   Hit *hits;                         // raw-hit ring
   unsigned long long *hit_count;     // slots requested (may exceed cap)
   uint64_t hit_cap;
-  // null, or 4 words: the shader clock (s_memtime) and the 100 MHz real-time
-  // clock (s_memrealtime) when the launch's first wave starts, then when it
-  // ends -- the clock the walk ran at
-  unsigned long long *clk;
 };
 
 struct FilterArgs {

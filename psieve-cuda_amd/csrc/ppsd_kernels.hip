@@ -24,11 +24,6 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   const uint32_t count = cnt < a.cap ? cnt : a.cap;
   const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * R;
   if (first >= count) return;
-  const bool timer = a.clk && blockIdx.x == 0 && threadIdx.x == 0;
-  if (timer) {
-    a.clk[0] = __builtin_amdgcn_s_memtime();
-    a.clk[1] = __builtin_amdgcn_s_memrealtime();
-  }
   LanePrimes<R> L;
   LaneStates<R> S;
 #pragma unroll
@@ -40,10 +35,6 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   start_states<SIGN, R>(L, S, a.nmin, a.s, rule.bound);
   const RingSink sink{a.hits, a.hit_count, a.hit_cap};
   walk_windows<SIGN, R, U, S32, PRE, PRE == PRE_RANGE>(L, S, a.nmin, a.nwin, rule, sink);
-  if (timer) {
-    a.clk[2] = __builtin_amdgcn_s_memtime();
-    a.clk[3] = __builtin_amdgcn_s_memrealtime();
-  }
 }
 
 template <int SIGN, int R, bool S32, int PRE>

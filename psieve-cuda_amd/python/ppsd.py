@@ -47,7 +47,7 @@ class Stats(C.Structure):
         ("window_steps", C.c_uint64), ("raw_hits", C.c_uint64),
         ("factors", C.c_uint64), ("walk_launches", C.c_uint64),
         ("walk_ms", C.c_double), ("sieve_ms", C.c_double), ("filter_ms", C.c_double),
-        ("retries", C.c_uint64), ("clock_mhz", C.c_double),
+        ("retries", C.c_uint64),
     ]
 
 
