@@ -154,6 +154,36 @@ def cpu_baseline_port(threads, target_s):
                       f"in {dt:.1f} s on {threads} threads"}
 
 
+# Non-move VALU instructions per prime-window of the walk loops' pre-tests,
+# as compiled (one sign, both signs): odd-low v_ffbl_b32 + v_alignbit_b32 +
+# half a v_min3_u32; exact v_ffbl_b32 + v_lshrrev_b64 + v_cmp_ge_u64; range
+# half a v_min3_u32 (and half a v_max3_u32); both signs add p - K.
+PRETEST_NONMOVE = {1: (3.0, 8.0), 2: (0.5, 1.0), 3: (2.5, 7.0)}
+
+
+def walk_issue_seconds(params, primes, r, u, batched, cus, clock_hz):
+    """A lower bound on the time any path needs to walk these units: per
+    unit, its prime-windows times the non-move VALU instructions per
+    prime-window of the loop it is walked with (the step's v_mul_lo_u32 and
+    two v_mad_u64_u32, plus v_lshlrev_b64 at s < 32 and, for the primes past
+    the three-multiply bound at the top of each 2^32 block, one more mad;
+    the pre-test of its form; one compare per chunk), over 64 lanes, against
+    one single-rate wave-instruction per CU per clock.  Prime supply and the
+    filter are left out."""
+    import ppsd
+
+    total = 0.0
+    for p, n in zip(params, primes):
+        form, chunk = ppsd.walk_form(p, r, u, batched)
+        both = p.sign == 0
+        wins = (p.nmax - p.nmin + p.nstep - 1) // p.nstep if p.nmax > p.nmin else 0
+        slow = min(1.0, ((p.pmin + p.pmax) // 2 >> p.nstep) / 2.0**32)
+        step = 3.0 + (p.nstep < 32) + slow
+        per = step + PRETEST_NONMOVE[form][1 if both else 0] + 1.0 / (r * chunk)
+        total += float(n) * wins * per / 64.0
+    return total / (cus * clock_hz)
+
+
 def mixed_units(n):
     """The first n units of the seeded mixed workload (4096 units, one
     persistent launch) searched twice: batched (ppsd_batch_run) and one unit
@@ -200,10 +230,13 @@ def mixed_units(n):
     bu, bf = canon(unit, out)
     identical = same_counts and len(pu) == len(bu) and np.array_equal(pu, bu) and all(
         np.array_equal(pf[name], bf[name]) for name in ("p", "k", "n", "c"))
+    _, cus = ppsd.device_info(0)
+    limit = walk_issue_seconds(params, cnt, shape["r"], shape["u"], True, cus, ppsd.device_clock_mhz(0) * 1e6)
     return {"units": n, "seed": ppsd_units.MIX_SEED, "primes": st.primes, "window_steps": st.window_steps,
             "factors": len(out), "batched_s": tb, "plain_s": tp, "batched_primes_per_s": st.primes / tb,
             "plain_primes_per_s": st.primes / tp, "batched_over_plain": tp / tb, "identical_per_unit": bool(identical),
-            "batch_shape": shape}
+            "batch_shape": shape, "walk_issue_limit_s": limit, "batched_frac_of_limit": limit / tb,
+            "plain_frac_of_limit": limit / tp, "batched_over_plain_bound": tp / limit}
 
 
 def _is_prime64(n):

@@ -146,6 +146,12 @@ int ppsd_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *wg);
 int ppsd_set_pretest(ppsd_ctx *ctx, int mode);
 int ppsd_get_pretest(const ppsd_ctx *ctx, int *mode, int *plain_form, int *plain_u);
 
+/* The form (1, 2 or 3, as above) and chunk length the automatic choice
+ * gives a unit (parameters as ppsd_open takes them) at kernel shape (r, u):
+ * for the plain walk (batched 0), which may shorten the chunk below u, or
+ * the batched walk (batched 1).  No device is needed. */
+int ppsd_walk_form(const ppsd_params *p, int r, int u, int batched, int *form, int *chunk);
+
 /* Raw-hit ring capacity for later submissions (1 .. the allocated 2^24).
  * A batch whose raw hits overflow the ring is re-walked in halves, so the
  * capacity only trades memory for re-walks; the result never changes. */

@@ -352,6 +352,17 @@ int ppsd_get_pretest(const ppsd_ctx *c, int *mode, int *plain_form, int *plain_u
   return 0;
 }
 
+int ppsd_walk_form(const ppsd_params *p, int r, int u, int batched, int *form, int *chunk) {
+  if (!p) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
+  if (!walk_domain_ok(p)) return set_err(PPSD_ERR_INVALID_PARAM, "parameters outside the walk's domain");
+  if ((r != 1 && r != 2) || (u != 4 && u != 8 && u != 16))
+    return set_err(PPSD_ERR_INVALID_PARAM, "r must be 1 or 2, u 4, 8 or 16");
+  const WalkForm f = choose_form(*p, r, u, 0, !batched);
+  if (form) *form = f.pre == PRE_RANGE ? 2 : f.pre == PRE_ODDLO ? 3 : 1;
+  if (chunk) *chunk = f.u;
+  return 0;
+}
+
 int ppsd_set_hit_capacity(ppsd_ctx *c, uint64_t cap) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
   if (cap < 1 || cap > c->hit_alloc) return set_err(PPSD_ERR_INVALID_PARAM, "capacity must be 1..%llu", (unsigned long long)c->hit_alloc);

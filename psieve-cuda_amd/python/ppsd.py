@@ -94,6 +94,7 @@ def _load():
         "ppsd_set_hit_capacity": (C.c_int, [C.c_void_p, C.c_uint64]),
         "ppsd_set_pretest": (C.c_int, [C.c_void_p, C.c_int]),
         "ppsd_get_pretest": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
+        "ppsd_walk_form": (C.c_int, [_P(Params), C.c_int, C.c_int, C.c_int, _P(C.c_int), _P(C.c_int)]),
         "ppsd_set_candidates": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64]),
         "ppsd_get_stats": (C.c_int, [C.c_void_p, _P(Stats)]),
         "ppsd_reset_stats": (C.c_int, [C.c_void_p]),
@@ -198,6 +199,14 @@ def device_clock_mhz(device: int = 0) -> int:
     mhz = C.c_uint32()
     _check(lib().ppsd_device_clock(device, C.byref(mhz)), "ppsd_device_clock")
     return mhz.value
+
+
+def walk_form(params: Params, r: int, u: int, batched: bool = False):
+    """(form: 1 exact / 2 range / 3 odd-low, chunk length) the automatic
+    choice gives a unit at kernel shape (r, u)."""
+    f, c = C.c_int(), C.c_int()
+    _check(lib().ppsd_walk_form(C.byref(params), r, u, 1 if batched else 0, C.byref(f), C.byref(c)), "ppsd_walk_form")
+    return f.value, c.value
 
 
 def _ptr(a: np.ndarray):

@@ -93,3 +93,23 @@ def test_derive_rejects_exponents_past_31_bits():
     p = ppsd.manual_params(2**50, 2**50 + 10**6, 3, 9999, 100, 2**31, nstep=32)
     h = ctypes.c_void_p()
     assert ppsd.lib().ppsd_open(0, ctypes.byref(p), 0, ctypes.byref(h)) == ppsd.PPSD_ERR_INVALID_PARAM
+
+
+def test_walk_form_choice_without_a_device():
+    """The automatic pre-test choice (ppsd_walk_form): the range form for the
+    2^61 both-signs search with kmax 1e5 on the plain walk's longest chunk,
+    the odd-low form for the 2^50 search with kmax 9999, the exact form at
+    2^46 with kmax 1e6; the batched walk keeps the chunk it is given."""
+    cfg3 = ppsd.derive(ppsd.SearchParams(pmin=2**61, pmax=2**61 + 2 * 10**9, kmin=3, kmax=100001, nmin=100000,
+                                         nmax=10000000, sign=0))
+    cfg2 = ppsd.derive(ppsd.SearchParams(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000,
+                                         nmax=6000000))
+    big_k = ppsd.derive(ppsd.SearchParams(pmin=2**46, pmax=2**46 + 10**6, kmin=3, kmax=999999, nmin=100000,
+                                          nmax=2000000))
+    assert ppsd.walk_form(cfg3, 1, 16) == (2, 16)
+    assert ppsd.walk_form(cfg3, 1, 8) == (2, 8)
+    assert ppsd.walk_form(cfg2, 1, 16) == (3, 16)
+    assert ppsd.walk_form(big_k, 1, 16)[0] == 1
+    assert ppsd.walk_form(cfg3, 2, 8, batched=True)[1] == 8
+    with pytest.raises(ppsd.PPSDError):
+        ppsd.walk_form(cfg3, 3, 16)
