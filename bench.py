@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mix-units", type=int, default=256,
                     help="units of the seeded mixed workload run batched and plain (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="seconds of Poisson arrivals at 200 requests/s through the serving loop (0 = skip)")
     return ap.parse_args()
 
 
@@ -237,6 +239,19 @@ def mixed_units(n):
             "plain_primes_per_s": st.primes / tp, "batched_over_plain": tp / tb, "identical_per_unit": bool(identical),
             "batch_shape": shape, "walk_issue_limit_s": limit, "batched_frac_of_limit": limit / tb,
             "plain_frac_of_limit": limit / tp, "batched_over_plain_bound": tp / limit}
+
+
+def serving(seconds):
+    """The serving loop under the mixed workload's Poisson arrivals at 200
+    requests/s (3.8x what one MI355X serves) for `seconds`, no drain: the
+    requests completed, their latency percentiles (arrival to done, and its
+    queueing and service parts) and the device's throughput meanwhile."""
+    import ppsd_serve
+
+    rep, _ = ppsd_serve.run(rate=200.0, duration=seconds, drain=0.0)
+    keep = ("requests", "completed", "backlog", "latency_s", "queue_s", "service_s", "primes_per_s",
+            "window_steps_per_s", "busy_fraction", "slices", "mem_peak_bytes")
+    return dict({k: rep[k] for k in keep}, offered_rate_per_s=200.0, duration_s=seconds)
 
 
 def _is_prime64(n):
@@ -403,6 +418,8 @@ def main():
     }
     if a.mix_units:
         line["config"]["batched"] = mixed_units(a.mix_units)
+    if a.serve_seconds > 0:
+        line["config"]["serving"] = serving(a.serve_seconds)
     if not a.no_cpu_baseline:
         threads, nproc, model = host_cpu()
         host = {"nproc": nproc, "cpu_model": model, "threads_used": threads}

@@ -221,7 +221,8 @@ int ppsd_batch_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
  * arrival order and are admitted while the device memory they hold -- their
  * share of the resident prime pool (primes of their next p-piece, at most a
  * quarter of the pool) plus a hit-ring reservation -- fits mem_budget.
- * The device then runs slices of about slice_ms: each slice is one round of
+ * The device then runs slices of about slice_ms (twice that while requests
+ * wait for admission): each slice is one round of
  * the batched walk over one p-piece of every admitted request, the slice's
  * work shared equally among them, smallest remaining work first.  A request
  * arriving during a slice joins the next one.  Each request's result equals
@@ -231,7 +232,7 @@ typedef struct ppsd_server ppsd_server;
 
 typedef struct ppsd_serve_config {
   uint64_t mem_budget; /* bytes admitted requests may hold (0 = pool + rings, about 0.8 GiB) */
-  double slice_ms;     /* target length of one slice (0 = 50) */
+  double slice_ms;     /* target length of one slice (0 = 50); twice that while requests wait */
   uint32_t max_active; /* admitted requests at most (0 = 4096; <= 65536) */
   int32_t raw;         /* 1: raw divisors instead of filtered factors */
 } ppsd_serve_config;
