@@ -12,7 +12,8 @@ oracle's independent sieve; --steps K times a prefix of K steps.
 Prints one JSON line:  value = primes/s over the timed steps (inputs
 generated on the device, nothing crosses PCIe but the factors),
 roofline = the walk kernel's single-rate VALU issue against the chip's
-(CUs x 64 lanes x the peak shader clock), cpu_baseline = the reference's
+(CUs x 64 lanes x the peak shader clock; frac_at_measured_clock at the
+clock the firmware reports meanwhile), cpu_baseline = the reference's
 own CPU walk (its host program re-walking every prime with test_one_p) on
 the host's cores over a bounded sample of the same workload, with the
 oracle's C port of it beside it.
@@ -36,7 +37,8 @@ WORKLOAD = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=
 # lanes for 4 clocks, so a CU issues at most 64 single-rate lane-instructions
 # per clock.  The peak is CUs x 64 x the device's peak clock (2400 MHz on
 # MI355X), which bounds the fraction by 1; under this load the chip runs
-# below its peak clock, which the counter profile's cycle counts show.  The
+# below its peak clock, so the line also gives the fraction at the clock
+# the firmware reports during the timed steps (ClockSampler).  The
 # one v_mov_b32 per window retires beside the counted instructions and is
 # not counted.
 LANES_PER_CU_CLOCK = 64
@@ -74,6 +76,65 @@ def host_cpu():
     except OSError:
         pass
     return max(1, min(share, 32)), os.cpu_count(), model
+
+
+class ClockSampler:
+    """The shader clock the device's firmware reports while the timed steps
+    run: amdsmi's GPU metrics (current_gfxclks, one reading per XCD), read
+    every 50 ms on a thread.  The firmware refreshes them about once a
+    second, so the median over the readings is taken, which drops a stale
+    idle reading at the start.  The GPU is the first one amdsmi lists (the
+    only one on a one-GPU host).  Without amdsmi the readings stay empty."""
+
+    def __init__(self, period_s=0.05):
+        self.period_s = period_s
+        self.readings = []
+        self._stop = None
+        self._thread = None
+        self._smi = None
+
+    def __enter__(self):
+        import threading
+
+        try:
+            import amdsmi
+
+            amdsmi.amdsmi_init()
+            handle = amdsmi.amdsmi_get_processor_handles()[0]
+        except Exception as e:  # no library or no device access: no measured clock
+            print(f"no clock readings: {e}", file=sys.stderr)
+            return self
+        self._smi = amdsmi
+        self._stop = threading.Event()
+
+        def run():
+            while not self._stop.is_set():
+                try:
+                    clks = amdsmi.amdsmi_get_gpu_metrics_info(handle)["current_gfxclks"]
+                    vals = [c for c in clks if isinstance(c, int) and c > 0]
+                    if vals:
+                        self.readings.append(sum(vals) / len(vals))
+                except Exception:
+                    pass
+                self._stop.wait(self.period_s)
+
+        self._thread = threading.Thread(target=run, daemon=True)
+        self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        if self._thread:
+            self._stop.set()
+            self._thread.join()
+        if self._smi:
+            self._smi.amdsmi_shut_down()
+        return False
+
+    def median_mhz(self):
+        if not self.readings:
+            return None
+        v = sorted(self.readings)
+        return v[len(v) // 2]
 
 
 def parse():
@@ -243,7 +304,7 @@ def mixed_units(n):
 
 def serving(seconds):
     """The serving loop under the mixed workload's Poisson arrivals at 200
-    requests/s (3.8x what one MI355X serves) for `seconds`, no drain: the
+    requests/s (about 3.4x what one MI355X serves) for `seconds`, no drain: the
     requests completed, their latency percentiles (arrival to done, and its
     queueing and service parts) and the device's throughput meanwhile."""
     import ppsd_serve
@@ -340,18 +401,19 @@ def main():
         p = WORKLOAD["pmin"]  # the whole range, warm-up batches included
     p_start = p
     factors, nsteps, count, psum, found = 0, 0, 0, 0, []
-    t0 = time.perf_counter()
-    while p < WORKLOAD["pmax"] and (full or nsteps < a.steps):
-        p = ctx.submit_range(p, WORKLOAD["pmax"])
-        f, c, s = ctx.collect()
-        factors += len(f)
-        found.append(f)
-        count += c
-        psum = (psum + s) & (2**64 - 1)
-        nsteps += 1
-        if nsteps % 50 == 0 or not full:
-            print(f"step {nsteps}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-    dt = time.perf_counter() - t0
+    with ClockSampler() as clock:
+        t0 = time.perf_counter()
+        while p < WORKLOAD["pmax"] and (full or nsteps < a.steps):
+            p = ctx.submit_range(p, WORKLOAD["pmax"])
+            f, c, s = ctx.collect()
+            factors += len(f)
+            found.append(f)
+            count += c
+            psum = (psum + s) & (2**64 - 1)
+            nsteps += 1
+            if nsteps % 50 == 0 or not full:
+                print(f"step {nsteps}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+        dt = time.perf_counter() - t0
     st = ctx.stats()
     ctx.close()
     checksum = None
@@ -372,6 +434,7 @@ def main():
     nonmove = nonmove_per_window(shape["r"], shape["u"])
     achieved = steps_per_launch * nonmove / t_launch / 1e12  # T lane-instructions/s
     peak = cus * LANES_PER_CU_CLOCK * ppsd.device_clock_mhz(0) * 1e6 / 1e12
+    run_mhz = clock.median_mhz()
     model_tops = steps_per_launch * MODEL_OPS_PER_STEP / t_launch / 1e12
     line = {
         "metric": METRIC,
@@ -411,6 +474,9 @@ def main():
             "frac": achieved / peak if peak else None,
             "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
             "peak_clock_mhz": ppsd.device_clock_mhz(0),
+            "measured_clock_mhz": run_mhz,
+            "clock_readings": len(clock.readings),
+            "frac_at_measured_clock": achieved / (cus * LANES_PER_CU_CLOCK * run_mhz * 1e-6) if run_mhz else None,
             "cus": cus,
             "nonmove_valu_per_window": nonmove,
             "model_ratio_15op": model_tops / MODEL_PEAK_TOPS,

@@ -48,7 +48,7 @@ def test_seeded_mix_prefix_batched_vs_oracle():
 
 def test_poisson_serving_200_per_s_vs_oracle():
     """Config-5 arrivals: Poisson at 200 requests/s for 1 s (the mix's
-    stream, 3.8x the device's capacity), admitted by the memory budget and
+    stream, about 3.4x the device's capacity), admitted by the memory budget and
     searched in 50 ms slices while others arrive; after a drain every
     request has completed and each result passes the oracle checks."""
     rep, rec = ppsd_serve.run(rate=200.0, duration=1.0, drain=240.0, slice_ms=50.0, keep_factors=True)
