@@ -275,8 +275,9 @@ int ppsd_serve_close(ppsd_server *s);
  * ppsd_collect (raw == 0) or ppsd_collect_raw (raw != 0), on primes and an
  * output buffer that are already in device memory, and it allocates
  * nothing: its raw-hit ring (PPSD_NOALLOC_RING_HITS records), counters and
- * the filter's small-prime table live in a fixed device array of the
- * library (about 12.6 MB per GPU, held until the process ends).
+ * the filter's small-prime table with its powers of two live in a fixed
+ * device array of the library (about 15.4 MB per GPU, held until the
+ * process ends).
  *
  * P: n device-resident primes, odd in [pmin, 2^62), any order, n <=
  * PPSD_NOALLOC_MAX_PRIMES (more is PPSD_ERR_INVALID_PARAM).  out: a device

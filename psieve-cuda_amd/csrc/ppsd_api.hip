@@ -49,6 +49,23 @@ std::vector<SmallPrime> small_prime_table() {
   return sm;
 }
 
+// Powers of two modulo each prime of that table, in the layout divides_tab
+// reads (kPowPer u16 per prime): 2^j mod q for j < kPowLo, then
+// 2^(kPowLo*j) mod q for j < kPowHi.
+std::vector<uint16_t> pow2_table(const std::vector<SmallPrime> &sm) {
+  std::vector<uint16_t> t(sm.size() * kPowPer);
+  for (size_t i = 0; i < sm.size(); ++i) {
+    const uint32_t q = sm[i].q;
+    uint16_t *row = &t[i * kPowPer];
+    uint32_t x = 1;
+    for (uint32_t j = 0; j < kPowLo; ++j, x = 2 * x % q) row[j] = (uint16_t)x;
+    const uint32_t big = x;  // 2^kPowLo mod q
+    x = 1;
+    for (uint32_t j = 0; j < kPowHi; ++j, x = x * big % q) row[kPowLo + j] = (uint16_t)x;
+  }
+  return t;
+}
+
 }  // namespace ppsd
 
 static int lg2(uint64_t v) {
@@ -162,6 +179,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_hits);
   (void)hipFree(c->d_kept);
   (void)hipFree(c->d_small);
+  (void)hipFree(c->d_pow2);
   (void)hipFree(c->d_spinv);
   (void)hipFree(c->d_sp);
   (void)hipFree(c->d_bitmap);
@@ -302,6 +320,9 @@ int ppsd_open(int device, const ppsd_params *p, uint32_t batch, ppsd_ctx **out) 
     c->nsmall = (uint32_t)sm.size();
     OPEN_TRY(hipMalloc(&c->d_small, sm.size() * sizeof(SmallPrime)));
     OPEN_TRY(hipMemcpy(c->d_small, sm.data(), sm.size() * sizeof(SmallPrime), hipMemcpyHostToDevice));
+    const std::vector<uint16_t> pw = pow2_table(sm);
+    OPEN_TRY(hipMalloc(&c->d_pow2, pw.size() * sizeof(uint16_t)));
+    OPEN_TRY(hipMemcpy(c->d_pow2, pw.data(), pw.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   }
   // Sieving primes: every odd q <= isqrt(pmax-1), or only those <= -Q when
   // given (main.c:971-973; primes up to 61 are always used, sieve.c's
@@ -446,6 +467,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.kstep = c->prm.kstep;
   f.koffset = c->prm.koffset;
   f.small_primes = c->d_small;
+  f.pow2 = c->d_pow2;
   f.nsmall = c->nsmall;
   f.kept = c->d_kept;
   f.kept_count = c->d_counters + 1;
@@ -456,7 +478,7 @@ static int enqueue_walk(ppsd_ctx *c, uint32_t first, uint32_t n, const uint32_t 
   f.cand_n0 = c->cand_n0;
   f.cand_rows = c->cand_rows;
   HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  HIP_TRY(launch_filter(f, 2048, c->stream));
+  HIP_TRY(launch_filter(f, kFilterGrid, c->stream));
   HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   c->st.walk_launches++;
   return 0;
@@ -848,6 +870,7 @@ static BatchArgs batch_args(ppsd_ctx *c, int buf) {
   a.kept_count = c->b_counts + 1;
   a.kept_cap = c->b_hit_cap;
   a.small_primes = c->d_small;
+  a.pow2 = c->d_pow2;
   a.nsmall = c->nsmall;
   return a;
 }
@@ -875,7 +898,7 @@ static int round_graph(ppsd_ctx *c, int buf) {
   hipError_t e = launch_zero_u64(c->b_counts, 2, c->stream);
   if (e == hipSuccess) e = launch_zero_u32(c->b_next, 1, c->stream);
   if (e == hipSuccess) e = launch_persist(a, c->b_r, c->b_u, grid, c->stream);
-  if (e == hipSuccess) e = launch_batch_filter(a, 4096, c->stream);
+  if (e == hipSuccess) e = launch_batch_filter(a, kFilterGrid, c->stream);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (e != hipSuccess) return set_err(PPSD_ERR_FAILURE, "HIP error capturing the round: %s", hipGetErrorString(e));
@@ -1026,6 +1049,7 @@ int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, s
       sa.nq_small = std::min(c->nsp_small, sa.nq);
       sa.bitmap = c->d_bitmap;
       sa.qinv = c->d_spinv;
+      sa.block = 256;  // fits beside the persistent walk's waves
       HIP_TRY(launch_sieve(sa, st));
       HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_blocksum, c->b_pool[buf],
                                (uint32_t)c->b_pool_cap,

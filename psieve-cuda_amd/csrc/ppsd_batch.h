@@ -48,6 +48,7 @@ struct BatchArgs {
   unsigned long long *kept_count;
   uint64_t kept_cap;
   const SmallPrime *small_primes;  // padded to a multiple of 64
+  const uint16_t *pow2;            // their powers of two (ppsd_filter.h kPowPer each)
   uint32_t nsmall;
 };
 

@@ -104,10 +104,12 @@ struct PieceRule {
   }
 };
 
-__global__ __launch_bounds__(256) void batch_filter_kernel(BatchArgs a) {
+__global__ __launch_bounds__(kFilterThreads) void batch_filter_kernel(BatchArgs a) {
+  __shared__ __align__(16) uint16_t pass_pow[kPowPassBytes / sizeof(uint16_t)];
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
-  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.nsmall, PieceRule{a.pieces});
+  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.pow2, a.nsmall,
+              PieceRule{a.pieces}, pass_pow);
 }
 
 // Copies the running prime count and sum after a piece was sieved.
@@ -150,7 +152,7 @@ hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipSt
 }
 
 hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(batch_filter_kernel, dim3(grid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(batch_filter_kernel, dim3(grid), dim3(kFilterThreads), 0, st, a);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 
 #include "../../include/ppsd.h"
 #include "ppsd_batch.h"
+#include "ppsd_filter.h"
 #include "ppsd_kernels.h"
 #include "ppsd_walk.h"
 
@@ -24,6 +25,7 @@ int set_err(int code, const char *fmt, ...);
 uint64_t isqrt64(uint64_t x);
 std::vector<uint32_t> host_primes(uint32_t lim);
 std::vector<SmallPrime> small_prime_table();
+std::vector<uint16_t> pow2_table(const std::vector<SmallPrime> &sm);
 
 enum class Pending { none, primes, range };
 
@@ -88,6 +90,7 @@ struct ppsd_ctx {
   uint64_t hit_cap = 0;        // capacity in use
   uint64_t hit_alloc = 0;      // allocated entries
   ppsd::SmallPrime *d_small = nullptr;  // odd primes 11 <= q < 32768, padded to a multiple of 64
+  uint16_t *d_pow2 = nullptr;           // their powers of two, for the filter
   uint32_t nsmall = 0;
   uint32_t *d_sp = nullptr;     // sieving primes <= qmax
   uint32_t nsp = 0, nsp_small = 0;

@@ -17,22 +17,29 @@ __device__ __forceinline__ uint32_t mod_barrett(uint32_t x, uint32_t d, uint32_t
   return min(r, r - d);
 }
 
-// q | k*2^n + c ?  (k < 2^62, q an odd prime below 2^15)  k mod q from its
-// two 32-bit halves; 2^n mod q as 2^e with e = n mod (q-1) (Fermat), its top
-// four bits at once -- 2^(e>>11) < 2q since e < q-1 -- then eleven
-// square-and-maybe-double steps, each product below 2q^2 < 2^31.
-__device__ __forceinline__ bool divides(uint64_t k, uint32_t n, int32_t c, const SmallPrime &sp) {
+// Powers of two modulo each small prime q of the filter's table, u16 each,
+// kPowPer per prime in the table's order: 2^j mod q for j < kPowLo, then
+// 2^(kPowLo*j) mod q for j < kPowHi.  Since e = n mod (q-1) < 2^15 =
+// kPowLo*kPowHi, 2^e mod q is the product of one entry of each half.
+constexpr uint32_t kPowLoBits = 7;
+constexpr uint32_t kPowLo = 1u << kPowLoBits, kPowHi = 256, kPowPer = kPowLo + kPowHi;
+constexpr uint32_t kPowPassBytes = 64 * kPowPer * sizeof(uint16_t);  // one pass of 64 primes: 48 KiB
+
+// q | k*2^n + c ?  (q an odd prime below 2^15.)  k mod q, from both 32-bit
+// halves only when BIGK (the high half times 2^32 mod q); 2^n mod q as 2^e
+// with e = n mod (q-1) (Fermat), from the two halves of q's powers (row, in
+// LDS): every product is below q^2 < 2^30.  wantv is the residue k*2^n must
+// have: q - 1 for c = +1, 1 for c = -1.  About 30 VALU instructions, where
+// eleven square-and-maybe-double steps for 2^e took about 125.
+template <bool BIGK>
+__device__ __forceinline__ bool divides_tab(uint32_t klo, uint32_t khi, uint32_t n, uint32_t wantv,
+                                            const SmallPrime &sp, const uint16_t *row) {
   const uint32_t q = sp.q;
-  const uint32_t kl = mod_barrett((uint32_t)k, q, sp.mq);
-  const uint32_t kh = mod_barrett((uint32_t)(k >> 32), q, sp.mq);
-  const uint32_t km = mod_barrett(kh * sp.c32 + kl, q, sp.mq);
+  uint32_t km = mod_barrett(klo, q, sp.mq);
+  if (BIGK) km = mod_barrett(mod_barrett(khi, q, sp.mq) * sp.c32 + km, q, sp.mq);
   const uint32_t e = mod_barrett(n, q - 1, sp.md);
-  uint32_t t = 1u << (e >> 11);
-  t = min(t, t - q);
-#pragma unroll
-  for (int b = 10; b >= 0; --b) t = mod_barrett((t * t) << ((e >> b) & 1u), q, sp.mq);
-  const uint32_t v = mod_barrett(km * t, q, sp.mq);
-  return v == (c > 0 ? q - 1 : 1u);
+  const uint32_t t = mod_barrett((uint32_t)row[e & (kPowLo - 1)] * row[kPowLo + (e >> kPowLoBits)], q, sp.mq);
+  return mod_barrett(km * t, q, sp.mq) == wantv;
 }
 
 // The residue pre-tests of test_factor in wrapping 64-bit arithmetic with their
@@ -74,23 +81,6 @@ __device__ __forceinline__ bool rule_cheap(uint64_t k, uint32_t n, int32_t c, ui
   return keep;
 }
 
-// try_all_factors for one (k, n, c) held by every lane of the wave: the
-// lanes split the small primes, 64 per pass, smallest first; most composites
-// fall in the first passes, so the wave stops at the first pass that finds a
-// divisor.  Each pass loads the next pass's constants before it computes.
-// nsmall is a multiple of 64.  Returns the same value in every lane.
-__device__ __forceinline__ bool no_small_factor(uint64_t k, uint32_t n, int32_t c, const SmallPrime *small_primes,
-                                                uint32_t nsmall) {
-  const uint32_t lane = __lane_id();
-  SmallPrime cur = small_primes[lane];
-  for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
-    const SmallPrime next = small_primes[j0 + 64 < nsmall ? j0 + 64 + lane : lane];
-    if (__ballot(divides(k, n, c, cur)) != 0) return false;
-    cur = next;
-  }
-  return true;
-}
-
 // Slots for the lanes set in keep (the same mask in every lane) at the end
 // of a ring, in lane order, with one atomic per wave: a counter every kept
 // hit incremented on its own serialised the filter on hit-dense units.
@@ -105,40 +95,81 @@ __device__ __forceinline__ unsigned long long wave_slot(uint64_t keep, unsigned 
   return base + (unsigned long long)__popcll(keep & ((1ull << lane) - 1));
 }
 
-// Filters raw hits [0, nraw) into a kept ring.  Each wave takes 64 hits at a
-// time, one per lane, applies the cheap part of the rule lane-parallel, then
-// runs the trial division wave-wide for each survivor in turn; a kept hit is
-// written by the lane that holds it.  rule(x, kstep, koffset, nmin, nmax)
-// gives the hit's rule.
+// Workgroup shape of the filter kernels: kFilterThreads threads, each
+// holding kFilterSlots hits, and one pass of powers of two in LDS (48 KiB:
+// three workgroups fit a CU's 160 KiB, 24 waves).
+constexpr uint32_t kFilterThreads = 512;
+constexpr int kFilterSlots = 4;
+constexpr uint32_t kFilterGrid = 768;  // workgroups per launch: three per CU of MI355X's 256
+
+// Filters raw hits [0, nraw) into a kept ring.  A workgroup takes
+// kFilterThreads * kFilterSlots hits at a time (kFilterSlots per lane) and
+// applies the cheap part of the rule lane-parallel.  try_all_factors then
+// runs pass by pass over the small primes, 64 per pass, smallest first: the
+// workgroup stages the pass's powers of two (pow2, kPowPer per prime) in LDS
+// (lds: kPowPassBytes, 16-byte aligned), and each wave tests its surviving
+// hits one at a time against the pass's 64 primes, one prime per lane.  A hit
+// with a divisor drops out; most composites do in the first passes, and the
+// workgroup stops when none of its hits is left.  A kept hit is written by
+// the lane that holds it.  rule(x, kstep, koffset, nmin, nmax) gives the
+// hit's rule; nsmall is a multiple of 64.  Every thread of the workgroup
+// must call this (it synchronises the workgroup).
 template <class HitT, class Rule>
 __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, HitT *kept,
                                             unsigned long long *kept_count, uint64_t kept_cap,
-                                            const SmallPrime *small_primes, uint32_t nsmall, const Rule &rule) {
+                                            const SmallPrime *small_primes, const uint16_t *pow2, uint32_t nsmall,
+                                            const Rule &rule, uint16_t *lds) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t base = wave * 64; base < nraw; base += nwaves * 64) {
-    const uint64_t h = base + lane;
-    const bool valid = h < nraw;
-    HitT x{};
-    if (valid) x = hits[h];
-    uint32_t kstep = 2, koffset = 1, nmin = 0, nmax = 0;
-    if (valid) rule(x, kstep, koffset, nmin, nmax);
-    const bool cand = valid && rule_cheap(x.k, x.n, x.c, kstep, koffset, nmin, nmax);
-    uint64_t mask = __ballot(cand);
-    uint64_t keepmask = 0;  // wave-uniform: each verdict below is the wave's
-    while (mask) {
-      const int j = __ffsll((unsigned long long)mask) - 1;
-      mask &= mask - 1;
-      // the hit of lane j, read into scalar registers (j is wave-uniform)
-      const uint64_t k = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x.k >> 32), j) << 32) |
-                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x.k, j);
-      const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)x.n, j);
-      const int32_t c = __builtin_amdgcn_readlane((int)x.c, j);
-      if (no_small_factor(k, n, c, small_primes, nsmall)) keepmask |= 1ull << j;
+  const uint64_t per = (uint64_t)blockDim.x * kFilterSlots;
+  const uint64_t first = (uint64_t)(threadIdx.x >> 6) * 64 * kFilterSlots + lane;
+  const uint16_t *row = lds + lane * kPowPer;  // this lane's prime's powers, in every pass
+  for (uint64_t base = blockIdx.x * per; base < nraw; base += (uint64_t)gridDim.x * per) {  // workgroup-uniform
+    HitT x[kFilterSlots];
+    bool alive[kFilterSlots];
+#pragma unroll
+    for (int s = 0; s < kFilterSlots; ++s) {
+      const uint64_t h = base + first + 64 * s;
+      const bool valid = h < nraw;
+      x[s] = HitT{};
+      if (valid) x[s] = hits[h];
+      uint32_t kstep = 2, koffset = 1, nmin = 0, nmax = 0;
+      if (valid) rule(x[s], kstep, koffset, nmin, nmax);
+      alive[s] = valid && rule_cheap(x[s].k, x[s].n, x[s].c, kstep, koffset, nmin, nmax);
     }
-    const unsigned long long idx = wave_slot(keepmask, kept_count);
-    if (((keepmask >> lane) & 1) && idx < kept_cap) kept[idx] = x;
+    for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
+      __syncthreads();  // the previous pass's table is no longer read
+      const uint4 *src = (const uint4 *)(pow2 + (size_t)j0 * kPowPer);
+      uint4 *dst = (uint4 *)lds;
+      for (uint32_t i = threadIdx.x; i < kPowPassBytes / 16; i += blockDim.x) dst[i] = src[i];
+      const SmallPrime sp = small_primes[j0 + lane];
+      bool any = false;
+#pragma unroll
+      for (int s = 0; s < kFilterSlots; ++s) any |= alive[s];
+      if (!__syncthreads_or(any)) break;  // also the barrier after staging
+#pragma unroll
+      for (int s = 0; s < kFilterSlots; ++s) {
+        uint64_t mask = __ballot(alive[s]);
+        while (mask) {
+          const int j = __ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+          // the hit of lane j, read into scalar registers (j is wave-uniform)
+          const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(x[s].k >> 32), j);
+          const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x[s].k, j);
+          const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)x[s].n, j);
+          const int32_t c = __builtin_amdgcn_readlane((int)x[s].c, j);
+          const uint32_t wantv = c > 0 ? sp.q - 1 : 1u;
+          const bool dv = khi ? divides_tab<true>(klo, khi, n, wantv, sp, row)
+                              : divides_tab<false>(klo, khi, n, wantv, sp, row);
+          if (__ballot(dv) != 0 && (int)lane == j) alive[s] = false;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kFilterSlots; ++s) {
+      const uint64_t keepmask = __ballot(alive[s]);
+      const unsigned long long idx = wave_slot(keepmask, kept_count);
+      if (alive[s] && idx < kept_cap) kept[idx] = x[s];
+    }
   }
 }
 

@@ -48,6 +48,7 @@ struct FilterArgs {
   uint32_t nmin, nmax;
   uint32_t kstep, koffset;
   const SmallPrime *small_primes;  // odd primes 11 <= q < 32768, padded
+  const uint16_t *pow2;            // their powers of two (ppsd_filter.h kPowPer each)
   uint32_t nsmall;                 // a multiple of 64
   Hit *kept;
   unsigned long long *kept_count;
@@ -67,6 +68,7 @@ struct SieveArgs {
   uint32_t nq;       // q[nq_small..nq) are marked with global atomics
   uint32_t *bitmap;  // ceil(nbits/32) words; set bit = composite
   const uint64_t *qinv;  // floor((2^64-1)/q) for q[0..nq_small), or null
+  uint32_t block;    // threads per LDS-sieve workgroup: 1024 (default) or 256
 };
 
 // Segment geometry of the LDS sieve: 2^18 candidates (2^19 integers) per

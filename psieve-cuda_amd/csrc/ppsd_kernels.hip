@@ -95,11 +95,12 @@ struct PlainRule {
   }
 };
 
-__global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
+__global__ __launch_bounds__(kFilterThreads) void filter_kernel(FilterArgs a) {
+  __shared__ __align__(16) uint16_t pass_pow[kPowPassBytes / sizeof(uint16_t)];
   const unsigned long long raw = *a.hit_count;
   const uint64_t nraw = raw < a.hit_cap ? raw : a.hit_cap;
-  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.nsmall,
-              PlainRule{a.kstep, a.koffset, a.nmin, a.nmax});
+  filter_hits(a.hits, nraw, a.kept, a.kept_count, a.kept_cap, a.small_primes, a.pow2, a.nsmall,
+              PlainRule{a.kstep, a.koffset, a.nmin, a.nmax}, pass_pow);
 }
 
 // With a sieve file's candidate list the rule is membership alone
@@ -136,7 +137,7 @@ hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st) {
   if (a.cand)
     hipLaunchKernelGGL(filter_cand_kernel, dim3(grid), dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kFilterThreads), 0, st, a);
   return hipGetLastError();
 }
 
@@ -169,8 +170,12 @@ __device__ __forceinline__ uint64_t first_multiple_inv(uint64_t from, uint32_t q
 // One workgroup per segment of kSegBits candidates.  Primes below 1024 are
 // marked cooperatively by the whole workgroup (each has many multiples in a
 // segment); the rest of q[0..nq_small) one prime per thread.  Every bitmap
-// word of the segment is written, tail bits past nbits set.
-__global__ __launch_bounds__(1024) void sieve_small_kernel(SieveArgs a) {
+// word of the segment is written, tail bits past nbits set.  BS threads:
+// 1024 where the sieve has the chip to itself; 256 (one wave per SIMD) where
+// it runs beside a persistent walk that leaves one wave slot per SIMD free,
+// which a 1024-thread workgroup (four waves per SIMD) could never get.
+template <int BS>
+__global__ __launch_bounds__(BS) void sieve_small_kernel(SieveArgs a) {
   __shared__ uint32_t seg[kSegWords];
   const uint64_t b0 = (uint64_t)blockIdx.x * kSegBits;
   const uint64_t rem = a.nbits - b0;
@@ -217,7 +222,10 @@ __global__ __launch_bounds__(256) void sieve_large_kernel(SieveArgs a) {
 hipError_t launch_sieve(const SieveArgs &a, hipStream_t st) {
   const uint32_t nseg = (uint32_t)((a.nbits + kSegBits - 1) / kSegBits);
   // 1024 threads: a segment's block walks every LDS-sieved prime (up to 2^21)
-  hipLaunchKernelGGL(sieve_small_kernel, dim3(nseg), dim3(1024), 0, st, a);
+  if (a.block == 256)
+    hipLaunchKernelGGL(sieve_small_kernel<256>, dim3(nseg), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(sieve_small_kernel<1024>, dim3(nseg), dim3(1024), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.nq > a.nq_small) {

@@ -15,14 +15,16 @@ using namespace ppsd;
 
 namespace {
 
-// Layout of the scratch array: counters, the filter's table, the raw ring.
+// Layout of the scratch array: counters, the filter's table and its powers
+// of two, the raw ring.
 constexpr size_t kCountersOff = 0;  // u64: raw slots requested, kept slots requested, bad primes
 constexpr size_t kTableOff = 256;
 constexpr size_t kTableMax = 3584;  // SmallPrime entries (3520 used: primes 11..32749 padded to 64s)
-constexpr size_t kRingOff = kTableOff + kTableMax * sizeof(SmallPrime);
+constexpr size_t kPowOff = kTableOff + kTableMax * sizeof(SmallPrime);
+constexpr size_t kRingOff = kPowOff + kTableMax * kPowPer * sizeof(uint16_t);
 constexpr uint64_t kRingHits = PPSD_NOALLOC_RING_HITS;
 constexpr size_t kScratchBytes = kRingOff + kRingHits * sizeof(Hit);
-static_assert(kRingOff % 256 == 0, "ring alignment");
+static_assert(kPowOff % 256 == 0 && kRingOff % 256 == 0, "table and ring alignment");
 
 }  // namespace
 
@@ -60,6 +62,8 @@ int setup(int device, uint8_t **ptr, uint32_t *nsmall) {
     const std::vector<SmallPrime> sm = small_prime_table();
     if (sm.size() > kTableMax) return set_err(PPSD_ERR_FAILURE, "small-prime table exceeds its scratch slot");
     HIP_TRY(hipMemcpy((uint8_t *)p + kTableOff, sm.data(), sm.size() * sizeof(SmallPrime), hipMemcpyHostToDevice));
+    const std::vector<uint16_t> pw = pow2_table(sm);
+    HIP_TRY(hipMemcpy((uint8_t *)p + kPowOff, pw.data(), pw.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     g_nsmall[device] = (uint32_t)sm.size();
     g_ptr[device] = (uint8_t *)p;
   }
@@ -128,6 +132,7 @@ int ppsd_walk_primes_noalloc(int device, const ppsd_params *p, const uint64_t *P
   f.kstep = p->kstep;
   f.koffset = p->koffset;
   f.small_primes = table;
+  f.pow2 = (const uint16_t *)(base + kPowOff);
   f.nsmall = nsmall;
   f.kept = (Hit *)out;         // factors go straight to the caller's buffer
   f.kept_count = counters + 1;  // running over every piece: the next free slot
@@ -172,7 +177,7 @@ int ppsd_walk_primes_noalloc(int device, const ppsd_params *p, const uint64_t *P
                                hipMemcpyDeviceToDevice, st));
       total += nraw;
     } else if (nraw) {
-      HIP_TRY(launch_filter(f, 2048, st));
+      HIP_TRY(launch_filter(f, kFilterGrid, st));
     }
   }
   if (!raw) HIP_TRY(hipMemcpyAsync(&total, counters + 1, sizeof total, hipMemcpyDeviceToHost, st));
