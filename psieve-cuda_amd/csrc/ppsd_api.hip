@@ -192,13 +192,18 @@ static int free_ctx(ppsd_ctx *c) {
     if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
     (void)hipFree(c->b_pool[i]);
     (void)hipFree(c->b_pool_count[i]);
-    (void)hipFree(c->b_pool_sum[i]);
     (void)hipFree(c->b_off[i]);
     (void)hipFree(c->b_sums[i]);
     for (auto &e : c->ev2[i])
       if (e) (void)hipEventDestroy(e);
     if (c->h_off[i]) (void)hipHostFree(c->h_off[i]);
     if (c->h_sums[i]) (void)hipHostFree(c->h_sums[i]);
+    (void)hipFree(c->d_rpieces[i]);
+    (void)hipFree(c->d_rsegs[i]);
+    (void)hipFree(c->d_rorder[i]);
+    if (c->h_rorder[i]) (void)hipHostFree(c->h_rorder[i]);
+    if (c->h_rpieces[i]) (void)hipHostFree(c->h_rpieces[i]);
+    if (c->h_rsegs[i]) (void)hipHostFree(c->h_rsegs[i]);
   }
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   (void)hipFree(c->b_pieces);
@@ -209,9 +214,13 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->b_kept);
   (void)hipFree(c->b_out);
   (void)hipFree(c->b_counts);
+  (void)hipFree(c->b_rbitmap);
+  (void)hipFree(c->b_rblockcount);
+  (void)hipFree(c->b_rblocksum);
   if (c->h_pieces) (void)hipHostFree(c->h_pieces);
   if (c->h_tasks) (void)hipHostFree(c->h_tasks);
   if (c->h_ntasks) (void)hipHostFree(c->h_ntasks);
+  if (c->h_bstage) (void)hipHostFree(c->h_bstage);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -798,14 +807,25 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   for (int i = 0; i < 2; ++i) {
     HIP_TRY(hipMalloc(&c->b_pool[i], c->b_pool_cap * sizeof(uint64_t)));
     HIP_TRY(hipMalloc(&c->b_pool_count[i], sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&c->b_pool_sum[i], sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&c->b_off[i], (c->b_piece_cap + 1) * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&c->b_sums[i], (c->b_piece_cap + 1) * sizeof(unsigned long long)));
     for (auto &e : c->ev2[i]) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipHostMalloc((void **)&c->h_off[i], (c->b_piece_cap + 1) * sizeof(uint32_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_sums[i], (c->b_piece_cap + 1) * sizeof(unsigned long long),
                           hipHostMallocDefault));
+    // a round holds at most b_piece_cap pieces and fits the pool: its
+    // segments number at most its bits / kSegBits + one per piece
+    c->b_rseg_cap = 2 * c->b_piece_cap;
+    HIP_TRY(hipMalloc(&c->d_rpieces[i], c->b_piece_cap * sizeof(RoundPiece)));
+    HIP_TRY(hipMalloc(&c->d_rsegs[i], c->b_rseg_cap * sizeof(uint32_t)));
+    HIP_TRY(hipHostMalloc((void **)&c->h_rpieces[i], c->b_piece_cap * sizeof(RoundPiece), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_rsegs[i], c->b_rseg_cap * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&c->d_rorder[i], c->b_piece_cap * sizeof(uint32_t)));
+    HIP_TRY(hipHostMalloc((void **)&c->h_rorder[i], c->b_piece_cap * sizeof(uint32_t), hipHostMallocDefault));
   }
+  HIP_TRY(hipMalloc(&c->b_rbitmap, (size_t)kRoundSegs * kSegWords * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_rblockcount, (size_t)kRoundSegs * kBlocksPerSeg * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&c->b_rblocksum, (size_t)kRoundSegs * kBlocksPerSeg * sizeof(unsigned long long)));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIP_TRY(hipMalloc(&c->b_pieces, c->b_piece_cap * sizeof(PieceDesc)));
   HIP_TRY(hipMalloc(&c->b_tasks, (size_t)c->b_task_cap * sizeof(Task)));
@@ -814,8 +834,12 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipMalloc(&c->b_hits, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_kept, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_counts, 2 * sizeof(unsigned long long)));
-  c->b_out_cap = c->b_hit_alloc;
+  // results stay on the device until ppsd_batch_results: room for 2^26
+  // (2 GiB), flushed to the host mid-batch only beyond that
+  c->b_out_cap = (uint64_t)1 << 26;
   HIP_TRY(hipMalloc(&c->b_out, c->b_out_cap * sizeof(BHit)));
+  c->h_bstage_cap = (uint64_t)1 << 20;
+  HIP_TRY(hipHostMalloc((void **)&c->h_bstage, c->h_bstage_cap * sizeof(BHit), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
@@ -1027,40 +1051,90 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int b
   return 0;
 }
 
-// Sieve pieces [b, e) back to back into pool `buf` on stream st, with a
-// snapshot of the running count and sum after each piece (asynchronous).
+// Sieve pieces [b, e) back to back into pool `buf` on stream st
+// (asynchronous): the pieces' segments go to one bitmap, sieved, compacted
+// and totalled per piece in one set of launches for as many pieces as its
+// kRoundSegs segments hold -- a handful of launches per round, where a chain
+// of small launches per piece had left the device mostly idle between walks.
+// b_off[buf][i] and b_sums[buf][i] receive each piece's pool offset and prime
+// sum, b_off[buf][np] the pool count.
 int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, size_t e, int buf, hipStream_t st,
                       hipEvent_t ev0, hipEvent_t ev1) {
   const uint32_t np = (uint32_t)(e - b);
-  HIP_TRY(launch_zero_u32(c->b_pool_count[buf], 1, st));
-  HIP_TRY(launch_zero_u64(c->b_pool_sum[buf], 1, st));
-  HIP_TRY(launch_snapshot(c->b_pool_count[buf], c->b_pool_sum[buf], c->b_off[buf], c->b_sums[buf], 0, st));
-  HIP_TRY(hipEventRecord(ev0, st));
+  if (np > c->b_piece_cap) return set_err(PPSD_ERR_FAILURE, "%u pieces in one round (at most %u)", np, c->b_piece_cap);
+  RoundPiece *rp = c->h_rpieces[buf];
+  uint32_t *segs = c->h_rsegs[buf];
+  // launches: [piece range, segment range) each within kRoundSegs segments
+  struct Launch {
+    uint32_t p0, p1, s0, s1;
+    uint32_t qhi;  // the largest nq of its pieces
+  };
+  std::vector<Launch> launches;
+  Launch cur{0, 0, 0, 0, 0};
+  uint32_t nseg_all = 0;
   for (uint32_t i = 0; i < np; ++i) {
     const Piece &P = pieces[b + i];
-    for (uint64_t lo = P.lo | 1; lo < P.hi;) {
-      uint64_t nbits = (P.hi - lo + 1) / 2;
-      nbits = std::min<uint64_t>(nbits, c->nbits_max);
-      SieveArgs sa{};
-      sa.lo = lo;
-      sa.nbits = nbits;
-      sa.q = c->d_sp;
-      sa.nq = sieve_count(c, lo + 2 * nbits);
-      sa.nq_small = std::min(c->nsp_small, sa.nq);
-      sa.bitmap = c->d_bitmap;
-      sa.qinv = c->d_spinv;
-      sa.block = 256;  // fits beside the persistent walk's waves
-      HIP_TRY(launch_sieve(sa, st));
-      HIP_TRY(launch_compact64(c->d_bitmap, lo, nbits, c->d_blockcount, c->d_blocksum, c->b_pool[buf],
-                               (uint32_t)c->b_pool_cap,
-                               c->b_pool_count[buf], c->b_pool_sum[buf], st));
-      lo += 2 * nbits;
+    const uint64_t lo = P.lo | 1;
+    const uint64_t nbits = P.hi > lo ? (P.hi - lo + 1) / 2 : 0;
+    const uint64_t ns = (nbits + kSegBits - 1) / kSegBits;
+    if (ns > kRoundSegs) return set_err(PPSD_ERR_FAILURE, "one piece needs %llu sieve segments", (unsigned long long)ns);
+    if (nseg_all + ns > c->b_rseg_cap) return set_err(PPSD_ERR_FAILURE, "round needs more than %u sieve segments", c->b_rseg_cap);
+    if ((nseg_all - cur.s0) + ns > kRoundSegs) {  // this piece starts the next launch
+      cur.p1 = i;
+      cur.s1 = nseg_all;
+      launches.push_back(cur);
+      cur = Launch{i, i, nseg_all, nseg_all, 0};
     }
-    HIP_TRY(launch_snapshot(c->b_pool_count[buf], c->b_pool_sum[buf], c->b_off[buf], c->b_sums[buf], i + 1, st));
+    RoundPiece &R = rp[i];
+    R.lo = lo;
+    R.nbits = nbits;
+    R.bit0 = (uint64_t)(nseg_all - cur.s0) * kSegBits;
+    R.nq = nbits ? sieve_count(c, lo + 2 * nbits) : 0;
+    R.nq_small = std::min(c->nsp_small, R.nq);
+    cur.qhi = std::max(cur.qhi, R.nq);
+    for (uint64_t k = 0; k < ns; ++k) segs[nseg_all++] = i;
   }
+  cur.p1 = np;
+  cur.s1 = nseg_all;
+  launches.push_back(cur);
+  // per launch, its pieces by descending nq: a large prime's thread stops at
+  // the first piece that does not sieve with it
+  uint32_t *order = c->h_rorder[buf];
+  for (const Launch &L : launches) {
+    for (uint32_t i = L.p0; i < L.p1; ++i) order[i] = i;
+    std::stable_sort(order + L.p0, order + L.p1, [&](uint32_t x, uint32_t y) { return rp[x].nq > rp[y].nq; });
+  }
+  HIP_TRY(hipEventRecord(ev0, st));
+  HIP_TRY(launch_zero_u32(c->b_pool_count[buf], 1, st));
+  if (np) {
+    HIP_TRY(hipMemcpyAsync(c->d_rpieces[buf], rp, np * sizeof(RoundPiece), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_rorder[buf], order, np * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (nseg_all)
+      HIP_TRY(hipMemcpyAsync(c->d_rsegs[buf], segs, nseg_all * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  }
+  for (const Launch &L : launches) {
+    if (L.p1 == L.p0) continue;
+    RoundSieveArgs a{};
+    a.pieces = c->d_rpieces[buf];
+    a.p0 = L.p0;
+    a.p1 = L.p1;
+    a.seg_piece = c->d_rsegs[buf] + L.s0;
+    a.nseg = L.s1 - L.s0;
+    a.by_nq = c->d_rorder[buf];
+    a.qlo = c->nsp_small;  // every piece with large primes marks q[0..nsp_small) in LDS
+    a.qhi = std::max(L.qhi, c->nsp_small);
+    a.q = c->d_sp;
+    a.qinv = c->d_spinv;
+    a.bitmap = c->b_rbitmap;
+    a.block = 256;  // one wave per SIMD: fits beside the persistent walk's waves
+    HIP_TRY(launch_round_sieve(a, c->b_rblockcount, c->b_rblocksum, c->b_pool[buf], (uint32_t)c->b_pool_cap,
+                               c->b_pool_count[buf], c->b_off[buf], c->b_sums[buf], st));
+  }
+  HIP_TRY(hipMemcpyAsync(c->b_off[buf] + np, c->b_pool_count[buf], sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemcpyAsync(c->h_off[buf], c->b_off[buf], (np + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(c->h_sums[buf], c->b_sums[buf], (np + 1) * sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, st));
+  if (np)
+    HIP_TRY(hipMemcpyAsync(c->h_sums[buf], c->b_sums[buf], np * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           st));
   HIP_TRY(hipEventRecord(ev1, st));
   return 0;
 }
@@ -1095,9 +1169,9 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     D.unit = P.unit;
     D.pre = choose_form(u, c->b_r, c->b_u, c->pre_mode, false).pre;
     c->b_unit_primes[P.unit] += D.nprimes;
-    c->b_unit_sums[P.unit] += sums[i + 1] - sums[i];
+    c->b_unit_sums[P.unit] += sums[i];
     c->st.primes += D.nprimes;
-    c->st.prime_sum += sums[i + 1] - sums[i];
+    c->st.prime_sum += sums[i];
     c->st.window_steps += (uint64_t)D.nprimes * D.nwin * (D.sign == 0 ? 2 : 1);
   }
   HIP_TRY(hipMemcpyAsync(c->b_pieces, c->h_pieces, np * sizeof(PieceDesc), hipMemcpyHostToDevice, c->stream));
@@ -1211,17 +1285,30 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
       return rc;
     }
   }
-  return batch_flush(c);
+  // the results stay where they are (b_result, then b_out) until
+  // ppsd_batch_results reads them
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int ppsd_batch_results(ppsd_ctx *c, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
-  if (n_out) *n_out = c->b_result.size();
-  const uint64_t n = std::min<uint64_t>(cap, c->b_result.size());
-  for (uint64_t i = 0; i < n; ++i) {
-    const BHit &h = c->b_result[i];
+  const uint64_t nhost = c->b_result.size();
+  if (n_out) *n_out = nhost + c->b_out_n;
+  const uint64_t n = std::min<uint64_t>(cap, nhost + c->b_out_n);
+  auto put = [&](uint64_t i, const BHit &h) {
     if (out) out[i] = ppsd_factor{h.p, h.k, h.n, h.c};
     if (unit_of) unit_of[i] = h.piece;
+  };
+  for (uint64_t i = 0; i < std::min(n, nhost); ++i) put(i, c->b_result[i]);
+  if (!out && !unit_of) return 0;
+  // the device part, through the pinned staging buffer
+  for (uint64_t i = nhost; i < n;) {
+    const uint64_t m = std::min(n - i, c->h_bstage_cap);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(c->h_bstage, c->b_out + (i - nhost), m * sizeof(BHit), hipMemcpyDeviceToHost));
+    for (uint64_t j = 0; j < m; ++j) put(i + j, c->h_bstage[j]);
+    i += m;
   }
   return 0;
 }

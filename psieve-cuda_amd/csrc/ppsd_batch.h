@@ -55,7 +55,5 @@ struct BatchArgs {
 hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipStream_t st);
 hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st);
 hipError_t launch_emit(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst, hipStream_t st);
-hipError_t launch_snapshot(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
-                           unsigned long long *sums, uint32_t i, hipStream_t st);
 
 }  // namespace ppsd

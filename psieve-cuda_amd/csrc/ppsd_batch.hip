@@ -112,15 +112,6 @@ __global__ __launch_bounds__(kFilterThreads) void batch_filter_kernel(BatchArgs 
               PieceRule{a.pieces}, pass_pow);
 }
 
-// Copies the running prime count and sum after a piece was sieved.
-__global__ void snapshot_kernel(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
-                                unsigned long long *sums, uint32_t i) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    off[i] = *count;
-    sums[i] = *sum;
-  }
-}
-
 // Appends n hits of a ring to the batch's output on the device, with the
 // round-local piece index of each replaced by the caller's unit index.
 __global__ __launch_bounds__(256) void emit_kernel(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst) {
@@ -159,12 +150,6 @@ hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st
 hipError_t launch_emit(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst, hipStream_t st) {
   const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
   if (blocks) hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, src, n, pieces, dst);
-  return hipGetLastError();
-}
-
-hipError_t launch_snapshot(const uint32_t *count, const unsigned long long *sum, uint32_t *off,
-                           unsigned long long *sums, uint32_t i, hipStream_t st) {
-  hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, st, count, sum, off, sums, i);
   return hipGetLastError();
 }
 

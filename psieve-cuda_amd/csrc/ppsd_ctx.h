@@ -127,7 +127,6 @@ struct ppsd_ctx {
   // current one is walked (on stream)
   uint64_t *b_pool[2] = {};
   uint32_t *b_pool_count[2] = {};
-  unsigned long long *b_pool_sum[2] = {};
   hipStream_t stream2 = nullptr;
   hipEvent_t ev2[2][2] = {};                  // per pool: sieve begin/end on stream2
   uint32_t *h_off[2] = {};                    // per pool, pinned: copies of b_off/b_sums
@@ -139,8 +138,16 @@ struct ppsd_ctx {
   BHit *b_hits = nullptr, *b_kept = nullptr;
   uint64_t b_hit_cap = 0, b_hit_alloc = 0;
   unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
-  uint32_t *b_off[2] = {};              // per pool: prime offset of each piece
-  unsigned long long *b_sums[2] = {};   // per pool: running prime sum after each piece
+  uint32_t *b_off[2] = {};              // per pool: prime offset of each piece (and the count after the last)
+  unsigned long long *b_sums[2] = {};   // per pool: prime sum of each piece
+  // a round's sieve (sieve_round): one bitmap of up to kRoundSegs segments
+  // per launch, and per pool the pieces and the segment -> piece table
+  uint32_t *b_rbitmap = nullptr, *b_rblockcount = nullptr;
+  unsigned long long *b_rblocksum = nullptr;
+  ppsd::RoundPiece *d_rpieces[2] = {}, *h_rpieces[2] = {};
+  uint32_t *d_rsegs[2] = {}, *h_rsegs[2] = {};
+  uint32_t *d_rorder[2] = {}, *h_rorder[2] = {};  // per launch, its pieces by descending nq
+  uint32_t b_rseg_cap = 0;
   hipGraph_t b_graph[2] = {};           // per pool: the round graph
   hipGraphExec_t b_exec[2] = {};
   int b_graph_key[2] = {-1, -1};
@@ -149,11 +156,16 @@ struct ppsd_ctx {
   BHit *b_out = nullptr;  // the batch's results so far, on the device, unit-indexed
   uint64_t b_out_cap = 0, b_out_n = 0;
   std::vector<BHit, ppsd::NoInitAlloc<BHit>> b_result;  // results copied back (batch_flush)
+  BHit *h_bstage = nullptr;  // pinned: b_out read back in pieces by ppsd_batch_results
+  uint64_t h_bstage_cap = 0;
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;
 };
 
 namespace ppsd {
+
+// Segments of one launch of a round's sieve: a 2^31-bit (256 MiB) bitmap.
+constexpr uint32_t kRoundSegs = 1u << 13;
 
 // Copies the device output of the batch so far (b_out) to b_result, after
 // the context's stream has finished.

@@ -76,11 +76,46 @@ struct SieveArgs {
 constexpr uint32_t kSegBits = 1u << 18;
 constexpr uint32_t kSegWords = kSegBits / 32;
 constexpr uint32_t kScanBlock = 256;  // words per compaction block
+constexpr uint32_t kBlocksPerSeg = kSegWords / kScanBlock;
+
+// One piece of a batched round's sieve: odd candidates lo + 2b, b < nbits,
+// held at bits bit0 + b of its launch's bitmap (bit0 a multiple of kSegBits,
+// so that each segment belongs to one piece).
+struct RoundPiece {
+  uint64_t lo, nbits, bit0;
+  uint32_t nq_small;  // sieving primes q[0..nq_small) marked in LDS
+  uint32_t nq;        // q[nq_small..nq) marked with global atomics
+};
+
+// The whole sieve of pieces [p0, p1) of a round in one set of launches,
+// whatever their number: the LDS sieve over every segment, the large primes
+// over every (piece, prime) pair, one compaction of the launch's bitmap.
+struct RoundSieveArgs {
+  const RoundPiece *pieces;   // the round's pieces
+  uint32_t p0, p1;            // this launch's
+  const uint32_t *seg_piece;  // per segment of this launch's bitmap: its piece
+  uint32_t nseg;
+  const uint32_t *by_nq;      // the launch's pieces in descending order of nq
+  uint32_t qlo, qhi;          // large primes of the launch: q[qlo..qhi), qhi = its largest nq
+  const uint32_t *q;          // sieving primes, ascending
+  const uint64_t *qinv;       // floor((2^64-1)/q) of the LDS-sieved ones
+  uint32_t *bitmap;           // nseg * kSegWords words; set bit = composite
+  uint32_t block;             // threads per LDS-sieve workgroup: 256 or 1024
+};
 
 hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block,
                        hipStream_t st);
 hipError_t launch_filter(const FilterArgs &a, uint32_t grid, hipStream_t st);
 hipError_t launch_sieve(const SieveArgs &a, hipStream_t st);
+// Sieve and compact one launch of a round (RoundSieveArgs): the primes go to
+// out (ascending per piece, pieces in order) from *count_d on; at most cap
+// entries of out are written, *count_d counts them all.  For each piece i of
+// the launch, off[i] receives the pool offset of its first prime and psum[i]
+// the sum of its primes mod 2^64.  blockcount and blocksum: one entry per
+// kScanBlock words of the bitmap.
+hipError_t launch_round_sieve(const RoundSieveArgs &a, uint32_t *blockcount, unsigned long long *blocksum,
+                              uint64_t *out, uint32_t cap, uint32_t *count_d, uint32_t *off,
+                              unsigned long long *psum, hipStream_t st);
 // Compaction: primes of the chunk to out (ascending), appended at *count_d;
 // at most cap entries of out are written, *count_d counts them all; *sum_d
 // accumulates their sum mod 2^64 through blocksum (one entry per kScanBlock

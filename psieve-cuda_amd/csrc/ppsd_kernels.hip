@@ -174,36 +174,80 @@ __device__ __forceinline__ uint64_t first_multiple_inv(uint64_t from, uint32_t q
 // 1024 where the sieve has the chip to itself; 256 (one wave per SIMD) where
 // it runs beside a persistent walk that leaves one wave slot per SIMD free,
 // which a 1024-thread workgroup (four waves per SIMD) could never get.
+// The body of one segment: candidates seg_lo + 2b, b < nb (nb <= kSegBits),
+// marked in LDS (seg) with q[0..nq_small) (qinv: their reciprocals, or
+// null); then nw_out words go to out, every bit past nb set.
+__device__ __forceinline__ void sieve_segment(uint32_t *seg, uint64_t seg_lo, uint32_t nb, const uint32_t *qs,
+                                              const uint64_t *qinv, uint32_t nq_small, uint32_t *out,
+                                              uint32_t nw_out) {
+  for (uint32_t w = threadIdx.x; w < kSegWords; w += blockDim.x) seg[w] = 0;
+  __syncthreads();
+  uint32_t i = 0;
+  for (; i < nq_small && qs[i] < 1024u; ++i) {
+    const uint32_t q = qs[i];
+    const uint64_t m = qinv ? first_multiple_inv(seg_lo, q, qinv[i]) : first_multiple(seg_lo, q);
+    const uint64_t off = (m - seg_lo) >> 1;
+    if (off >= nb) continue;
+    for (uint64_t b = off + (uint64_t)threadIdx.x * q; b < nb; b += (uint64_t)blockDim.x * q)
+      atomicOr(&seg[b >> 5], 1u << (b & 31));
+  }
+  for (uint32_t t = i + threadIdx.x; t < nq_small; t += blockDim.x) {
+    const uint32_t q = qs[t];
+    const uint64_t m = qinv ? first_multiple_inv(seg_lo, q, qinv[t]) : first_multiple(seg_lo, q);
+    for (uint64_t b = (m - seg_lo) >> 1; b < nb; b += q) atomicOr(&seg[b >> 5], 1u << (b & 31));
+  }
+  __syncthreads();
+  for (uint32_t w = threadIdx.x; w < nw_out; w += blockDim.x) {
+    uint32_t v = seg[w];
+    const uint32_t bb = w * 32;
+    if (bb >= nb) v = ~0u;
+    else if (bb + 32 > nb) v |= ~((1u << (nb - bb)) - 1u);
+    out[w] = v;
+  }
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void sieve_small_kernel(SieveArgs a) {
   __shared__ uint32_t seg[kSegWords];
   const uint64_t b0 = (uint64_t)blockIdx.x * kSegBits;
   const uint64_t rem = a.nbits - b0;
   const uint32_t nb = rem < kSegBits ? (uint32_t)rem : kSegBits;
-  const uint64_t seg_lo = a.lo + 2 * b0;
-  for (uint32_t w = threadIdx.x; w < kSegWords; w += blockDim.x) seg[w] = 0;
-  __syncthreads();
-  uint32_t i = 0;
-  for (; i < a.nq_small && a.q[i] < 1024u; ++i) {
-    const uint32_t q = a.q[i];
-    const uint64_t m = a.qinv ? first_multiple_inv(seg_lo, q, a.qinv[i]) : first_multiple(seg_lo, q);
-    const uint64_t off = (m - seg_lo) >> 1;
-    if (off >= nb) continue;
-    for (uint64_t b = off + (uint64_t)threadIdx.x * q; b < nb; b += (uint64_t)blockDim.x * q)
-      atomicOr(&seg[b >> 5], 1u << (b & 31));
-  }
-  for (uint32_t t = i + threadIdx.x; t < a.nq_small; t += blockDim.x) {
-    const uint32_t q = a.q[t];
-    const uint64_t m = a.qinv ? first_multiple_inv(seg_lo, q, a.qinv[t]) : first_multiple(seg_lo, q);
-    for (uint64_t b = (m - seg_lo) >> 1; b < nb; b += q) atomicOr(&seg[b >> 5], 1u << (b & 31));
-  }
-  __syncthreads();
-  const uint32_t nw = (nb + 31) / 32;
-  for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
-    uint32_t v = seg[w];
-    const uint32_t bb = w * 32;
-    if (bb + 32 > nb) v |= ~((1u << (nb - bb)) - 1u);
-    a.bitmap[b0 / 32 + w] = v;
+  sieve_segment(seg, a.lo + 2 * b0, nb, a.q, a.qinv, a.nq_small, a.bitmap + b0 / 32, (nb + 31) / 32);
+}
+
+// One workgroup per segment of a round launch: the segment's piece from
+// seg_piece; all kSegWords words are written (a piece's last segment is
+// padded with composites), so the compaction can run over whole blocks.
+template <int BS>
+__global__ __launch_bounds__(BS) void round_sieve_small_kernel(RoundSieveArgs a) {
+  __shared__ uint32_t seg[kSegWords];
+  const uint32_t g = blockIdx.x;
+  const RoundPiece P = a.pieces[a.seg_piece[g]];
+  const uint64_t b0 = (uint64_t)g * kSegBits - P.bit0;  // within the piece
+  const uint64_t rem = P.nbits - b0;
+  const uint32_t nb = rem < kSegBits ? (uint32_t)rem : kSegBits;
+  sieve_segment(seg, P.lo + 2 * b0, nb, a.q, a.qinv, P.nq_small, a.bitmap + (uint64_t)g * kSegWords, kSegWords);
+}
+
+// The large primes of a round launch, one per thread: q = q[t] for t in
+// [qlo, qhi), marked in every piece of the launch that sieves with it (those
+// whose nq exceeds t: a prefix of by_nq).  The reciprocal floor((2^64-1)/q)
+// is formed once per prime, so that each piece's first multiple costs a
+// multiply-high and one correction instead of a 64-bit remainder.
+__global__ __launch_bounds__(256) void round_sieve_large_kernel(RoundSieveArgs a) {
+  const uint64_t t = (uint64_t)a.qlo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t >= a.qhi) return;
+  const uint32_t q = a.q[t];
+  const uint64_t inv = ~0ull / q;
+  for (uint32_t j = a.p0; j < a.p1; ++j) {
+    const RoundPiece P = a.pieces[a.by_nq[j]];
+    if (P.nq <= t) break;
+    if (t < P.nq_small) continue;  // marked in LDS for this piece
+    const uint64_t m = first_multiple_inv(P.lo, q, inv);
+    for (uint64_t b = (m - P.lo) >> 1; b < P.nbits; b += q) {
+      const uint64_t bit = P.bit0 + b;
+      atomicOr(&a.bitmap[bit >> 5], 1u << (bit & 31));
+    }
   }
 }
 
@@ -357,6 +401,74 @@ hipError_t launch_compact64(const uint32_t *bitmap, uint64_t lo, uint64_t nbits,
 hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits, uint32_t *blockcount, uint32_t *out,
                             uint32_t cap, uint32_t *count_d, hipStream_t st) {
   return launch_compact<uint32_t>(bitmap, lo, nbits, blockcount, out, cap, count_d, nullptr, nullptr, st);
+}
+
+// compact_kernel over a round launch's bitmap: a block's candidates belong
+// to the piece of its segment.
+__global__ __launch_bounds__(256) void round_compact_kernel(RoundSieveArgs a, const uint32_t *blockoff, uint64_t *out,
+                                                            uint32_t cap, unsigned long long *blocksum) {
+  const uint32_t blk = blockIdx.x;
+  const RoundPiece P = a.pieces[a.seg_piece[blk / kBlocksPerSeg]];
+  const uint64_t idx = (uint64_t)blk * kScanBlock + threadIdx.x;
+  uint32_t w = ~a.bitmap[idx];
+  uint32_t total;
+  uint32_t pos = blockoff[blk] + block_exclusive_scan((uint32_t)__popc(w), total);
+  unsigned long long s = 0;
+  while (w) {
+    const uint32_t bit = (uint32_t)__builtin_ctz(w);
+    w &= w - 1;
+    const uint64_t x = P.lo + 2 * (idx * 32 + bit - P.bit0);
+    if (pos < cap) out[pos] = x;  // the count still advances: the host sees an overflow
+    ++pos;
+    s += x;
+  }
+  __shared__ unsigned long long wsum[4];
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) blocksum[blk] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Per piece of a round launch (one wave each): the pool offset of its first
+// prime and the sum of its primes, from the blocks of its segments.
+__global__ __launch_bounds__(64) void round_piece_totals_kernel(RoundSieveArgs a, const uint32_t *blockoff,
+                                                                const unsigned long long *blocksum, uint32_t nblocks,
+                                                                const uint32_t *count_d, uint32_t *off,
+                                                                unsigned long long *psum) {
+  const uint32_t i = a.p0 + blockIdx.x;
+  const RoundPiece P = a.pieces[i];
+  const uint32_t first = (uint32_t)(P.bit0 / (kScanBlock * 32));
+  const uint32_t nblk = (uint32_t)((P.nbits + kSegBits - 1) / kSegBits) * kBlocksPerSeg;
+  unsigned long long s = 0;
+  for (uint32_t j = __lane_id(); j < nblk; j += 64) s += blocksum[first + j];
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+  if (__lane_id() == 0) {
+    off[i] = first < nblocks ? blockoff[first] : *count_d;  // an empty piece at the end: the final count
+    psum[i] = s;
+  }
+}
+
+hipError_t launch_round_sieve(const RoundSieveArgs &a, uint32_t *blockcount, unsigned long long *blocksum,
+                              uint64_t *out, uint32_t cap, uint32_t *count_d, uint32_t *off,
+                              unsigned long long *psum, hipStream_t st) {
+  const uint32_t nblocks = a.nseg * kBlocksPerSeg;
+  if (a.nseg) {
+    if (a.block == 256)
+      hipLaunchKernelGGL(round_sieve_small_kernel<256>, dim3(a.nseg), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL(round_sieve_small_kernel<1024>, dim3(a.nseg), dim3(1024), 0, st, a);
+    if (a.qhi > a.qlo)
+      hipLaunchKernelGGL(round_sieve_large_kernel, dim3((a.qhi - a.qlo + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(256), 0, st, a.bitmap, nblocks * kScanBlock, blockcount);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, st, blockcount, nblocks, count_d);
+    hipLaunchKernelGGL(round_compact_kernel, dim3(nblocks), dim3(256), 0, st, a, (const uint32_t *)blockcount, out,
+                       cap, blocksum);
+  }
+  if (a.p1 > a.p0)
+    hipLaunchKernelGGL(round_piece_totals_kernel, dim3(a.p1 - a.p0), dim3(64), 0, st, a,
+                       (const uint32_t *)blockcount, (const unsigned long long *)blocksum, nblocks,
+                       (const uint32_t *)count_d, off, psum);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -10,6 +10,7 @@ import pytest
 
 import pps_oracle as orc
 import ppsd
+import ppsd_units
 from conftest import GOLDEN, read_lines, windowed_run
 from unitgen import _dense_unit, _oracle, _ref_unit
 
@@ -89,6 +90,44 @@ def test_batch_golden_ranges_together():
         assert sum(int(cnt[i]) for i in idx) == RUNS[tag]["count"], tag
         assert sum(int(sums[i]) for i in idx) % 2**64 == RUNS[tag]["sum"], tag
     assert [int(x) for x in rcnt] == [int(x) for x in cnt]
+
+
+def test_round_sieve_in_several_launches():
+    """9000 narrow units fit one round of the prime pool, but each takes at
+    least one sieve segment (2^18 candidates) of the round's bitmap, which
+    holds 2^13 segments per launch: the round is sieved in two launches.
+    Every unit's prime count and sum equal the oracle's sieve, and the
+    divisors of every 61st unit (and of the units on either side of the
+    launch boundary) equal the oracle walk."""
+    rng = random.Random(8000)
+    units = []
+    for _ in range(9000):
+        pmin = rng.randrange(10**12, 10**14)
+        kw = dict(pmin=pmin, pmax=pmin + rng.randrange(1000, 300000), kmin=3, kmax=rng.randrange(101, 3000),
+                  nmin=rng.randrange(100, 5000), sign=rng.choice((1, -1)))
+        kw["nmax"] = kw["nmin"] + rng.randrange(1, 2000)
+        units.append(ppsd_units.extended_params(**kw))
+    # the first unit whose segments start the second launch
+    segs, boundary = 0, None
+    for i, p in enumerate(units):
+        segs += (((p.pmax - (p.pmin | 1) + 1) // 2) + 2**18 - 1) // 2**18
+        if segs > 2**13 and boundary is None:
+            boundary = i
+    assert boundary is not None
+    with ppsd.Context(units[0]) as ctx:
+        out, unit, cnt, sums = ctx.batch_run(units, raw=True)
+    want_cs = orc.count_primes_many([(p.pmin, p.pmax) for p in units], threads=16)
+    for i in range(len(units)):
+        assert (int(cnt[i]), int(sums[i])) == want_cs[i], f"unit {i}: prime count/sum"
+    got = _per_unit(out, unit, len(units))
+    check = sorted(set(range(0, len(units), 61)) | {boundary - 1, boundary, boundary + 1})
+    for i in check:
+        p = units[i]
+        pr = orc.OrcParams(pmin=p.pmin, pmax=p.pmax, kmin=p.kmin, kmax=p.kmax, nmin=p.nmin, nmax=p.nmax,
+                           kstep=p.kstep, koffset=p.koffset, sign=p.sign, nstep_ref=p.nstep_ref, nstep=p.nstep,
+                           status=0)
+        want = orc.hit_set(orc.walk(pr, orc.primes(p.pmin, p.pmax), filter=False, threads=8))
+        assert got[i] == want, f"unit {i}: {len(got[i])} divisors, oracle {len(want)}"
 
 
 def test_batch_equals_plain_path():
