@@ -264,13 +264,7 @@ def mixed_units(n):
     ctx = ppsd.Context(params[0])
     ppsd_tune.apply_batch(ctx)  # the autotuner's cached shape for this device, when present
     shape = dict(zip(("r", "u", "waves", "seg_min", "seg_max"), ctx.batch_tuning()))
-    # first use: buffers, graph capture, and the sieving primes up to
-    # isqrt(max pmax) -- context state, as ppsd_open builds them for the plain
-    # path -- through a narrow slice at the top of the highest unit
-    top = max(params, key=lambda p: p.pmax)
-    probe = type(top).from_buffer_copy(top)
-    probe.pmin = max(top.pmin, top.pmax - 200000)
-    ctx.batch_run(params[:4] + [probe])
+    ppsd_units.warm_up(ctx, params)  # first use, outside the clock
     ctx.reset_stats()
     t0 = time.perf_counter()
     out, unit, cnt, sums = ctx.batch_run(params)

@@ -96,7 +96,7 @@ def tune_batch(device: int = 0, units: int = 256, log=None):
     params = [u.params for u in ppsd_units.mix_units(units)]
     rows = []
     with ppsd.Context(params[0], device=device) as ctx:
-        ctx.batch_run(params[:4])
+        ppsd_units.warm_up(ctx, params)
         for r, u, w, (smin, smax) in itertools.product(*BATCH_GRID.values()):
             ctx.set_batch_tuning(r, u, w, smin, smax)
             t0 = time.perf_counter()

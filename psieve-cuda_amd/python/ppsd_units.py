@@ -80,6 +80,18 @@ def draw(rng) -> Unit:
         return Unit(search, extended_params(**search), False, False)
 
 
+def warm_up(ctx, params):
+    """First use of a context's batched path for these units, outside any
+    timing: buffers, graph capture, and the sieving primes up to
+    isqrt(max pmax) -- context state, as ppsd_open builds them for the plain
+    path -- through the first four units and a narrow slice at the top of the
+    highest one."""
+    top = max(params, key=lambda p: p.pmax)
+    probe = type(top).from_buffer_copy(top)
+    probe.pmin = max(top.pmin, top.pmax - 200000)
+    ctx.batch_run(list(params[:4]) + [probe])
+
+
 def mix_units(n: int = MIX_UNITS, seed: int = MIX_SEED):
     """The first n units of the seeded stream (a prefix of a longer stream is
     the shorter stream)."""
@@ -97,15 +109,23 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--run", type=int, default=256, help="units of the seeded mix to search")
     ap.add_argument("--raw", action="store_true", help="raw divisors instead of filtered factors")
+    ap.add_argument("--tuned", action="store_true", help="the tuning cache's batch shape, and a warm-up run")
     a = ap.parse_args()
     units = mix_units(a.run)
     params = [u.params for u in units]
     with ppsd.Context(params[0]) as ctx:
+        if a.tuned:
+            import ppsd_tune
+
+            ppsd_tune.apply_batch(ctx)
+            warm_up(ctx, params)
+            ctx.reset_stats()
         t0 = time.perf_counter()
         out, unit, cnt, sums = ctx.batch_run(params, raw=a.raw)
         dt = time.perf_counter() - t0
         st = ctx.stats()
-    print(json.dumps({"units": a.run, "seed": MIX_SEED, "seconds": dt, "primes": st.primes,
+        shape = ctx.batch_tuning()
+    print(json.dumps({"units": a.run, "seed": MIX_SEED, "batch_shape": shape, "seconds": dt, "primes": st.primes,
                       "window_steps": st.window_steps, "raw_hits": st.raw_hits, "factors": len(out),
                       "walk_ms": st.walk_ms, "sieve_ms": st.sieve_ms, "walk_launches": st.walk_launches,
                       "primes_per_s": st.primes / dt, "ref_rejected_units": sum(not u.ref_domain for u in units),
