@@ -210,7 +210,9 @@ int ppsd_batch_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *waves_per_si
                           uint32_t *seg_max);
 
 /* Raw-hit ring capacity of the batched walk (1 .. 2^25); overflowing task
- * sets are re-run in halves, so results never depend on it. */
+ * sets are re-run in halves, so results never depend on it.  The batch's
+ * device output then holds at most twice this many results before they are
+ * moved to host memory (results do not depend on that either). */
 int ppsd_batch_set_hit_capacity(ppsd_ctx *ctx, uint64_t cap);
 
 /* ---- Serving loop (no counterpart in the reference).  For callers that

@@ -1037,9 +1037,11 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int b
       todo.push_back({f, k / 2});
       continue;
     }
-    // the results stay on the device until the batch ends or its output fills
+    // the results stay on the device until the batch ends or its output
+    // fills; the output is used up to twice the ring (all of it at the
+    // default ring), so that a small ring also exercises the flush
     const uint64_t take = raw ? cnt[0] : cnt[1];
-    if (c->b_out_n + take > c->b_out_cap) {
+    if (c->b_out_n + take > std::min(c->b_out_cap, 2 * c->b_hit_cap)) {
       int frc = batch_flush(c);
       if (frc) return frc;
     }

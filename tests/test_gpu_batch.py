@@ -158,7 +158,9 @@ def test_batch_ring_overflow_reruns():
         # halves where a group still overflowed
         st = ctx.stats()
         assert st.walk_launches - launches > 2 * launches
-    assert len(full) > 512
+    # more results than the device output holds with this ring (twice its
+    # capacity): the output was moved to host memory mid-batch
+    assert len(full) > 2 * 512
     assert sorted(zip(map(tuple, full.tolist()), fu.tolist())) == sorted(zip(map(tuple, small.tolist()), su.tolist()))
 
 
