@@ -1177,7 +1177,7 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     c->st.window_steps += (uint64_t)D.nprimes * D.nwin * (D.sign == 0 ? 2 : 1);
   }
   HIP_TRY(hipMemcpyAsync(c->b_pieces, c->h_pieces, np * sizeof(PieceDesc), hipMemcpyHostToDevice, c->stream));
-  std::vector<Task> tasks;
+  std::vector<Task> &tasks = c->b_task_list;  // its capacity is kept from round to round
   build_tasks(c, par, np, tasks);
   // task groups whose expected raw hits fill at most half the ring
   std::vector<double> hpw(np);  // expected hits per prime-window of each piece
@@ -1222,6 +1222,12 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   int rc = batch_alloc(c);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream2));  // nothing of an earlier, failed run still in flight
+  // both pools' round graphs up front (captured again only when the kernel
+  // shape or the ring changed), not at the first round that uses each
+  for (int buf = 0; buf < 2 && nunits; ++buf) {
+    rc = round_graph(c, buf);
+    if (rc) return rc;
+  }
   if (nunits) {
     rc = ensure_sieving(c, pmax_all);
     if (rc) return rc;

@@ -158,6 +158,7 @@ struct ppsd_ctx {
   std::vector<BHit, ppsd::NoInitAlloc<BHit>> b_result;  // results copied back (batch_flush)
   BHit *h_bstage = nullptr;  // pinned: b_out read back in pieces by ppsd_batch_results
   uint64_t h_bstage_cap = 0;
+  std::vector<Task> b_task_list;  // a round's tasks, built on the host (capacity kept)
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;
 };
