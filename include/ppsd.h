@@ -193,6 +193,14 @@ const char *ppsd_last_error(void);
  * have been opened with a partial sieve.  Counters go to ppsd_get_stats. */
 int ppsd_batch_run(ppsd_ctx *ctx, const ppsd_params *units, uint32_t nunits, int raw);
 
+/* The batched path's one-time set-up, done ahead of the first ppsd_batch_run
+ * that would otherwise do it: its device buffers (about 6 GiB) and pinned
+ * staging, the sieving primes for every unit with pmax <= pmax_all, and the
+ * round graphs of the current kernel shape and ring capacity.  Optional:
+ * ppsd_batch_run does whatever is missing.  No counterpart in the reference,
+ * whose host allocates per device in cuda_app_init (pps/appcu.cu:213-404). */
+int ppsd_batch_prepare(ppsd_ctx *ctx, uint64_t pmax_all);
+
 /* Hits of the last ppsd_batch_run: up to cap of them to out, with the index
  * of their unit in unit_of (either may be NULL); *n_out = their number. */
 int ppsd_batch_results(ppsd_ctx *ctx, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out);

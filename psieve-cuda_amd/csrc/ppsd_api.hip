@@ -1203,6 +1203,23 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
   return 0;
 }
 
+int ppsd_batch_prepare(ppsd_ctx *c, uint64_t pmax_all) {
+  if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
+  if (pmax_all < 3 || pmax_all > ((uint64_t)1 << 62)) return set_err(PPSD_ERR_INVALID_PARAM, "pmax must be 3..2^62");
+  if (c->prm.qmax && c->qmax < isqrt64(c->prm.pmax - 1))
+    return set_err(PPSD_ERR_STATE, "context opened with a partial sieve (qmax)");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = batch_alloc(c);
+  if (rc) return rc;
+  // both pools' round graphs (captured again only when the kernel shape or
+  // the ring changed), not at the first round that uses each
+  for (int buf = 0; buf < 2; ++buf) {
+    rc = round_graph(c, buf);
+    if (rc) return rc;
+  }
+  return ensure_sieving(c, pmax_all);
+}
+
 int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int raw) {
   if (!c || (!units && nunits)) return set_err(PPSD_ERR_INVALID_PARAM, "null argument");
   if (c->pending != Pending::none) return set_err(PPSD_ERR_STATE, "a submission is outstanding");
@@ -1222,14 +1239,8 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   int rc = batch_alloc(c);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream2));  // nothing of an earlier, failed run still in flight
-  // both pools' round graphs up front (captured again only when the kernel
-  // shape or the ring changed), not at the first round that uses each
-  for (int buf = 0; buf < 2 && nunits; ++buf) {
-    rc = round_graph(c, buf);
-    if (rc) return rc;
-  }
   if (nunits) {
-    rc = ensure_sieving(c, pmax_all);
+    rc = ppsd_batch_prepare(c, pmax_all);
     if (rc) return rc;
   }
   c->b_result.clear();

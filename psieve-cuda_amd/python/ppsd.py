@@ -102,6 +102,7 @@ def _load():
         "ppsd_last_error": (C.c_char_p, []),
         "ppsd_debug_state": (C.c_int, [C.c_int, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
         "ppsd_batch_run": (C.c_int, [C.c_void_p, _P(Params), C.c_uint32, C.c_int]),
+        "ppsd_batch_prepare": (C.c_int, [C.c_void_p, C.c_uint64]),
         "ppsd_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, _P(C.c_uint32), C.c_uint64, _u64p]),
         "ppsd_batch_unit_primes": (C.c_int, [C.c_void_p, _u64p, _u64p, C.c_uint32]),
         "ppsd_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
@@ -328,6 +329,10 @@ class Context:
         return np.concatenate(parts) if parts else np.zeros(0, dtype=FACTOR_DTYPE)
 
     # ---- batched search: many independent units in one persistent launch
+    def batch_prepare(self, pmax_all: int):
+        """The batched path's one-time set-up for units with pmax <= pmax_all."""
+        _check(lib().ppsd_batch_prepare(self._h, pmax_all), "ppsd_batch_prepare")
+
     def batch_run(self, units, raw: bool = False):
         """Search every unit (a sequence of Params) completely; returns
         (hits, unit index per hit, primes per unit, prime sum per unit)."""

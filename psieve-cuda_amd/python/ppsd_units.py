@@ -82,14 +82,11 @@ def draw(rng) -> Unit:
 
 def warm_up(ctx, params):
     """First use of a context's batched path for these units, outside any
-    timing: buffers, graph capture, and the sieving primes up to
-    isqrt(max pmax) -- context state, as ppsd_open builds them for the plain
-    path -- through the first four units and a narrow slice at the top of the
-    highest one."""
-    top = max(params, key=lambda p: p.pmax)
-    probe = type(top).from_buffer_copy(top)
-    probe.pmin = max(top.pmin, top.pmax - 200000)
-    ctx.batch_run(list(params[:4]) + [probe])
+    timing: ppsd_batch_prepare (buffers, the round graphs, the sieving
+    primes up to isqrt of the highest pmax -- context state, as ppsd_open
+    builds it for the plain path), then a run of the first four units."""
+    ctx.batch_prepare(max(p.pmax for p in params))
+    ctx.batch_run(list(params[:4]))
 
 
 def mix_units(n: int = MIX_UNITS, seed: int = MIX_SEED):

@@ -183,6 +183,10 @@ def test_batch_empty_and_errors():
     with ppsd.Context(params) as ctx:
         out, unit, cnt, _ = ctx.batch_run([])
         assert len(out) == 0 and len(cnt) == 0
+        with pytest.raises(ppsd.PPSDError) as e:
+            ctx.batch_prepare(2**62 + 1)
+        assert e.value.code == ppsd.PPSD_ERR_INVALID_PARAM
+        ctx.batch_prepare(2**62)  # the set-up for any unit; the runs below then do none of it
         # an empty n-range and an empty p-range
         e1 = ppsd.manual_params(pr.pmin, pr.pmax, pr.kmin, pr.kmax, 100, 100, nstep=8)
         e2 = ppsd.manual_params(pr.pmin, pr.pmin + 1, pr.kmin, pr.kmax, 100, 200, nstep=8)
