@@ -4,16 +4,19 @@ Workload (single large work unit, plain kernel): Proth sign (+1), odd k in
 [1201, 9999], n in [50000, 6000000), p over the primes of [2^50, 2^50+5e10),
 exact uint64 arithmetic.  One step = one batch: the primes of the next
 p-chunk are sieved on the device (2^21 primes by default), walked over the
-whole n-range, hits filtered; the step ends when the batch's factors are on
-the host.  By default the steps cover the whole p-range (1.44e9 primes,
-about 690 steps), whose prime count and sum are checked against the
-oracle's independent sieve; --steps K times a prefix of K steps.
+whole n-range, hits filtered, and the batch's factors brought to the host.
+Two library contexts are submitted to in turn, as the host program drives
+them, so the next batch is queued while the current one walks.  By default
+the steps cover the whole p-range (1.44e9 primes, about 690 steps), whose
+prime count and sum are checked against the oracle's independent sieve;
+--steps K times a prefix of K steps.
 
 Prints one JSON line:  value = primes/s over the timed steps (inputs
 generated on the device, nothing crosses PCIe but the factors),
-roofline = the walk kernel's single-rate VALU issue against the chip's
-(CUs x 64 lanes x the peak shader clock; frac_at_measured_clock at the
-clock the firmware reports meanwhile), cpu_baseline = the reference's
+roofline = the walk's single-rate VALU issue (its algorithmic instructions
+per launch over the time per launch in the timed region) against the
+chip's (CUs x 64 lanes x the peak shader clock; frac_at_measured_clock at
+the clock the firmware reports meanwhile), cpu_baseline = the reference's
 own CPU walk (its host program re-walking every prime with test_one_p) on
 the host's cores over a bounded sample of the same workload, with the
 oracle's C port of it beside it.
@@ -82,13 +85,19 @@ class ClockSampler:
     """The shader clock the device's firmware reports while the timed steps
     run: amdsmi's GPU metrics (current_gfxclks, one reading per XCD), read
     every 50 ms on a thread.  The firmware refreshes them about once a
-    second, so the median over the readings is taken, which drops a stale
-    idle reading at the start.  The GPU is the first one amdsmi lists (the
-    only one on a one-GPU host).  Without amdsmi the readings stay empty."""
+    second, so what is read during the first settle_s seconds can still
+    describe the idle GPU before the steps began: those readings are
+    dropped, and a run too short to leave min_span_s seconds of readings
+    after them reports no clock rather than a stale one.  The median over
+    the rest is taken.  The GPU is the first one amdsmi lists (the only one
+    on a one-GPU host).  Without amdsmi the readings stay empty."""
 
-    def __init__(self, period_s=0.05):
+    def __init__(self, period_s=0.05, settle_s=1.5, min_span_s=2.0):
         self.period_s = period_s
+        self.settle_s = settle_s
+        self.min_span_s = min_span_s
         self.readings = []
+        self._times = []
         self._stop = None
         self._thread = None
         self._smi = None
@@ -113,6 +122,7 @@ class ClockSampler:
                     clks = amdsmi.amdsmi_get_gpu_metrics_info(handle)["current_gfxclks"]
                     vals = [c for c in clks if isinstance(c, int) and c > 0]
                     if vals:
+                        self._times.append(time.perf_counter())
                         self.readings.append(sum(vals) / len(vals))
                 except Exception:
                     pass
@@ -130,11 +140,20 @@ class ClockSampler:
             self._smi.amdsmi_shut_down()
         return False
 
+    def settled(self):
+        """Readings taken after the first settle_s seconds, or [] when they
+        span less than min_span_s seconds."""
+        if not self._times:
+            return []
+        t0 = self._times[0] + self.settle_s
+        kept = [(t, r) for t, r in zip(self._times, self.readings) if t >= t0]
+        if not kept or kept[-1][0] - kept[0][0] < self.min_span_s:
+            return []
+        return [r for _, r in kept]
+
     def median_mhz(self):
-        if not self.readings:
-            return None
-        v = sorted(self.readings)
-        return v[len(v) // 2]
+        v = sorted(self.settled())
+        return v[len(v) // 2] if v else None
 
 
 def parse():
@@ -384,38 +403,64 @@ def main():
     import ppsd_tune
 
     params = ppsd.derive(ppsd.SearchParams(**WORKLOAD))
-    ctx = ppsd.Context(params, batch=a.batch)
-    if a.tune:
-        r, u, wg = (int(x) for x in a.tune.split(","))
-        ctx.set_tuning(r, u, wg)
-    else:
-        ppsd_tune.apply_plain(ctx)  # the autotuner's cached shape for this device, when present
-    shape = dict(zip(("r", "u", "wg"), ctx.tuning()))
+    # Two library contexts, submitted to in turn, as the host program drives
+    # them: the next batch is queued while the current one walks, so the
+    # GPU goes from one walk to the next without waiting on the host.
+    ctxs = [ppsd.Context(params, batch=a.batch) for _ in range(2)]
+    for ctx in ctxs:
+        if a.tune:
+            r, u, wg = (int(x) for x in a.tune.split(","))
+            ctx.set_tuning(r, u, wg)
+        else:
+            ppsd_tune.apply_plain(ctx)  # the autotuner's cached shape for this device, when present
+    shape = dict(zip(("r", "u", "wg"), ctxs[0].tuning()))
     p = WORKLOAD["pmin"]
-    for _ in range(a.warmup):
-        p = ctx.submit_range(p, WORKLOAD["pmax"])
-        ctx.collect()
-    ctx.reset_stats()
+    for i in range(max(a.warmup, 2) if a.warmup else 0):
+        p = ctxs[i % 2].submit_range(p, WORKLOAD["pmax"])
+        ctxs[i % 2].collect()
+    for ctx in ctxs:
+        ctx.reset_stats()
     full = a.steps == 0
     if full:
         p = WORKLOAD["pmin"]  # the whole range, warm-up batches included
     p_start = p
     factors, nsteps, count, psum, found = 0, 0, 0, 0, []
+    pending = []
+
+    def take():
+        nonlocal factors, count, psum, nsteps
+        f, c, s = pending.pop(0).collect()
+        factors += len(f)
+        found.append(f)
+        count += c
+        psum = (psum + s) & (2**64 - 1)
+        nsteps += 1
+        if nsteps % 50 == 0 or not full:
+            print(f"step {nsteps}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+
+    submitted = 0
     with ClockSampler() as clock:
         t0 = time.perf_counter()
-        while p < WORKLOAD["pmax"] and (full or nsteps < a.steps):
+        while p < WORKLOAD["pmax"] and (full or submitted < a.steps):
+            ctx = ctxs[submitted % 2]
             p = ctx.submit_range(p, WORKLOAD["pmax"])
-            f, c, s = ctx.collect()
-            factors += len(f)
-            found.append(f)
-            count += c
-            psum = (psum + s) & (2**64 - 1)
-            nsteps += 1
-            if nsteps % 50 == 0 or not full:
-                print(f"step {nsteps}: p < {p}, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+            pending.append(ctx)
+            submitted += 1
+            if len(pending) == 2:
+                take()
+        while pending:
+            take()
         dt = time.perf_counter() - t0
-    st = ctx.stats()
-    ctx.close()
+    sts = [ctx.stats() for ctx in ctxs]
+    for ctx in ctxs:
+        ctx.close()
+
+    class Sum:  # the two contexts' counters together
+        pass
+
+    st = Sum()
+    for name in ("primes", "window_steps", "walk_launches", "walk_ms", "sieve_ms", "filter_ms"):
+        setattr(st, name, sum(getattr(x, name) for x in sts))
     checksum = None
     if full:  # the reference's count= and sum= of the whole range, from the oracle's own sieve
         with open(os.path.join(HERE, "tests", "golden", "runs.json")) as fh:
@@ -429,7 +474,13 @@ def main():
                            params.kmin, params.kmax, params.nmin, params.nmax)
     value = st.primes / dt
     steps_per_launch = st.window_steps / max(st.walk_launches, 1)
-    t_launch = st.walk_ms / 1e3 / max(st.walk_launches, 1)
+    # time per launch in the steady state of the timed region: the launches
+    # follow one another with no gap, and the region's sieve, filter and host
+    # work (which overlap them) count against the walk, so this is an upper
+    # bound on the kernel's own time per launch.  HIP events around each
+    # launch (walk_ms) overlap between the two streams: their mean is given
+    # as launch_ms_events.
+    t_launch = dt / max(st.walk_launches, 1)
     _, cus = ppsd.device_info(0)
     nonmove = nonmove_per_window(shape["r"], shape["u"])
     achieved = steps_per_launch * nonmove / t_launch / 1e12  # T lane-instructions/s
@@ -464,6 +515,7 @@ def main():
             "sieve_ms": st.sieve_ms,
             "filter_ms": st.filter_ms,
             "parallelism": "1 GPU",
+            "contexts_in_flight": 2,
             "walk_shape": shape,
         },
         "roofline": {
@@ -472,10 +524,12 @@ def main():
             "peak": peak,
             "unit": "T single-rate lane-instructions/s",
             "frac": achieved / peak if peak else None,
+            "launch_ms": t_launch * 1e3,
+            "launch_ms_events": st.walk_ms / max(st.walk_launches, 1),
             "traffic": TRAFFIC_BYTES_PER_PRIME * st.primes / max(st.walk_launches, 1),
             "peak_clock_mhz": ppsd.device_clock_mhz(0),
             "measured_clock_mhz": run_mhz,
-            "clock_readings": len(clock.readings),
+            "clock_readings": len(clock.settled()),
             "frac_at_measured_clock": achieved / (cus * LANES_PER_CU_CLOCK * run_mhz * 1e-6) if run_mhz else None,
             "cus": cus,
             "nonmove_valu_per_window": nonmove,
