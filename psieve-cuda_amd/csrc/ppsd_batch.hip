@@ -27,11 +27,12 @@ namespace ppsd {
 
 // Raw-hit ring of the batched walk: 32-byte records tagged with the piece.
 struct PieceSink {
+  using Rec = BHit;
   BHit *hits;
   unsigned long long *count;
   uint64_t cap;
   uint32_t piece;
-  __device__ __forceinline__ void put(uint64_t idx, uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
+  __device__ __forceinline__ BHit make(uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
     BHit h;
     h.p = p;
     h.k = k;
@@ -39,12 +40,17 @@ struct PieceSink {
     h.c = c;
     h.piece = piece;
     h.pad = 0;
-    hits[idx] = h;
+    return h;
   }
 };
 
+// Hits staged per wave in LDS across tasks (each record carries its piece);
+// flushed when the stage fills and when the wave leaves.
+constexpr uint32_t kBatchStage = 64;
+
 template <int SIGN, int R, int U, bool S32, int PRE>
-__device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P, const Task &T) {
+__device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P, const Task &T, BHit *stage,
+                                         uint32_t *fill) {
   LanePrimes<R> L;
   LaneStates<R> S;
   const uint32_t first = T.first + __lane_id() * R;
@@ -57,38 +63,52 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P,
   const uint32_t n0 = P.nmin + T.w0 * P.s;
   const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s, P.kmax << (P.s - 1)};
   start_states<SIGN, R>(L, S, n0, P.s, rule.bound);
-  const PieceSink sink{a.hits, a.hit_count, a.hit_cap, T.piece};
+  const StagedSink<PieceSink, kBatchStage> sink{PieceSink{a.hits, a.hit_count, a.hit_cap, T.piece}, stage, fill};
   // states not kept: the persistent kernel holds every form, and its
   // registers (so its waves per SIMD) are set by the largest
   walk_windows<SIGN, R, U, S32, PRE, false>(L, S, n0, T.nw, rule, sink);
 }
 
 template <int R, int U, int PRE>
-__device__ __forceinline__ void run_form(const BatchArgs &a, const PieceDesc &P, const Task &T) {
+__device__ __forceinline__ void run_form(const BatchArgs &a, const PieceDesc &P, const Task &T, BHit *stage,
+                                         uint32_t *fill) {
   const bool s32 = P.s == 32;
   if (P.sign == 1) {
-    if (s32) run_task<1, R, U, true, PRE>(a, P, T); else run_task<1, R, U, false, PRE>(a, P, T);
+    if (s32) run_task<1, R, U, true, PRE>(a, P, T, stage, fill);
+    else run_task<1, R, U, false, PRE>(a, P, T, stage, fill);
   } else if (P.sign == -1) {
-    if (s32) run_task<-1, R, U, true, PRE>(a, P, T); else run_task<-1, R, U, false, PRE>(a, P, T);
+    if (s32) run_task<-1, R, U, true, PRE>(a, P, T, stage, fill);
+    else run_task<-1, R, U, false, PRE>(a, P, T, stage, fill);
   } else {
-    if (s32) run_task<0, R, U, true, PRE>(a, P, T); else run_task<0, R, U, false, PRE>(a, P, T);
+    if (s32) run_task<0, R, U, true, PRE>(a, P, T, stage, fill);
+    else run_task<0, R, U, false, PRE>(a, P, T, stage, fill);
   }
 }
 
-// Every wave pulls tasks until the queue is empty; all waves reach the exit.
+// Every wave pulls tasks until the queue is empty; all waves reach the exit,
+// flushing their staged hits first.
 template <int R, int U>
 __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
+  __shared__ BHit stage[4][kBatchStage];
+  __shared__ uint32_t fill[4];
+  const uint32_t wave = threadIdx.x >> 6;
+  fill[wave] = 0;
   const uint32_t ntasks = *a.ntasks;
   for (;;) {
     uint32_t t = 0;
     if (__lane_id() == 0) t = atomicAdd(a.next_task, 1u);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-    if (t >= ntasks) return;
+    if (t >= ntasks) {
+      const StagedSink<PieceSink, kBatchStage> sink{PieceSink{a.hits, a.hit_count, a.hit_cap, 0}, stage[wave],
+                                                    &fill[wave]};
+      sink.flush();
+      return;
+    }
     const Task T = a.tasks[t];
     const PieceDesc P = a.pieces[T.piece];
-    if (P.pre == PRE_RANGE) run_form<R, U, PRE_RANGE>(a, P, T);
-    else if (P.pre == PRE_ODDLO) run_form<R, U, PRE_ODDLO>(a, P, T);
-    else run_form<R, U, PRE_EXACT>(a, P, T);
+    if (P.pre == PRE_RANGE) run_form<R, U, PRE_RANGE>(a, P, T, stage[wave], &fill[wave]);
+    else if (P.pre == PRE_ODDLO) run_form<R, U, PRE_ODDLO>(a, P, T, stage[wave], &fill[wave]);
+    else run_form<R, U, PRE_EXACT>(a, P, T, stage[wave], &fill[wave]);
   }
 }
 

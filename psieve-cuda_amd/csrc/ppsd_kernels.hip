@@ -18,8 +18,12 @@ namespace ppsd {
 // ---------------------------------------------------------------------------
 // plain walk: one lane per R primes of one batch
 // ---------------------------------------------------------------------------
+constexpr uint32_t kPlainStage = 32;  // hits staged per wave in LDS
+
 template <int SIGN, int R, int U, bool S32, int PRE>
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
+  __shared__ Hit stage[4][kPlainStage];
+  __shared__ uint32_t fill[4];
   const uint32_t cnt = a.count_d ? *a.count_d : a.count;
   const uint32_t count = cnt < a.cap ? cnt : a.cap;
   const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * R;
@@ -33,8 +37,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
   }
   const WindowRule rule{a.kmin, a.kmax, a.nmax, a.s, a.kmax << (a.s - 1)};
   start_states<SIGN, R>(L, S, a.nmin, a.s, rule.bound);
-  const RingSink sink{a.hits, a.hit_count, a.hit_cap};
+  const uint32_t wave = threadIdx.x >> 6;
+  fill[wave] = 0;
+  const StagedSink<RingSink, kPlainStage> sink{RingSink{a.hits, a.hit_count, a.hit_cap}, stage[wave], &fill[wave]};
   walk_windows<SIGN, R, U, S32, PRE, PRE == PRE_RANGE>(L, S, a.nmin, a.nwin, rule, sink);
+  sink.flush();
 }
 
 template <int SIGN, int R, bool S32, int PRE>

@@ -41,36 +41,75 @@ __device__ __forceinline__ uint32_t odd_low(uint64_t K) {
 
 // Raw-hit ring of the plain walk: 24-byte records.
 struct RingSink {
+  using Rec = Hit;
   Hit *hits;
   unsigned long long *count;
   uint64_t cap;
-  __device__ __forceinline__ void put(uint64_t idx, uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
+  __device__ __forceinline__ Hit make(uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
     Hit h;
     h.p = p;
     h.k = k;
     h.n = n;
     h.c = c;
-    hits[idx] = h;
+    return h;
   }
 };
 
-// Ballot + prefix inside the wave, one atomic per wave; slots past the
-// ring's capacity are counted but not written (the host re-walks then).
-template <class Sink>
-__device__ __forceinline__ void emit(bool hit, uint64_t p, uint64_t k, uint32_t n, int32_t c, const Sink &sink) {
-  const uint64_t mask = __ballot(hit);
-  if (mask == 0) return;
-  const uint32_t lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  unsigned long long base = 0;
-  if ((int)lane == leader) base = atomicAdd(sink.count, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
-  if (hit) {
-    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-    const uint64_t idx = base + (uint64_t)__popcll(below);
-    if (idx < sink.cap) sink.put(idx, p, k, n, c);
+// Hits staged per wave in LDS and appended to a device ring CAP at a time.
+// Every wave of the chip adds to the ring's one counter; with one atomic per
+// window that holds a hit, a hit-dense search (tens of hits per prime) kept
+// its waves queued on that address and walked at about 0.8 of its issue
+// rate.  Staged, a wave takes one atomic per CAP hits.  Slots past the
+// ring's capacity are counted but not written (the host re-runs then).
+// `stage` and `fill` are this wave's LDS records and staged count; put()
+// and flush() are called by all the wave's active lanes together, and the
+// active set differs from call to call (a chunk is re-walked by the lanes
+// whose pre-test fired), so the count lives in LDS, where every lane sees
+// the last update: `fill` is volatile, or the compiler would keep a copy of
+// it per lane, and lanes left out of a call would keep a stale one.
+template <class Ring, uint32_t CAP>
+struct StagedSink {
+  Ring ring;
+  typename Ring::Rec *stage;
+  volatile uint32_t *fill;
+
+  __device__ __forceinline__ void flush_n(uint32_t f) const {
+    const uint64_t act = __ballot(true);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    unsigned long long base = 0;
+    if ((int)lane == leader) base = atomicAdd(ring.count, (unsigned long long)f);
+    base = __shfl(base, leader);
+    // the active lanes copy the f records between them, in rank order
+    const uint32_t nact = (uint32_t)__popcll(act);
+    const uint32_t rank = (uint32_t)__popcll(act & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    for (uint32_t i = rank; i < f; i += nact)
+      if (base + i < ring.cap) ring.hits[base + i] = stage[i];
   }
-}
+  __device__ __forceinline__ void put(bool hit, uint64_t p, uint64_t k, uint32_t n, int32_t c) const {
+    const uint64_t mask = __ballot(hit);
+    if (mask == 0) return;
+    const uint32_t nh = (uint32_t)__popcll(mask);
+    uint32_t f = *fill;
+    if (f + nh > CAP) {
+      flush_n(f);
+      f = 0;
+    }
+    if (hit) {
+      const uint32_t lane = __lane_id();
+      const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+      stage[f + (uint32_t)__popcll(below)] = ring.make(p, k, n, c);
+    }
+    *fill = f + nh;
+  }
+  __device__ __forceinline__ void flush() const {
+    const uint32_t f = *fill;
+    if (f) {
+      flush_n(f);
+      *fill = 0;
+    }
+  }
+};
 
 // Exact window test (pps/app.c:1093-1109 semantics) on a state reduced to
 // [0, p): K = k*2^i with i < s, kmin <= k <= kmax, n+i inside [nmin, nmax).
@@ -130,12 +169,12 @@ __device__ __forceinline__ void rescan(const LanePrimes<R> L, LaneStates<R> S, u
         uint32_t nn = 0;
         if (SIGN == 0) {
           bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
-          emit(h, L.p[r], k, nn, -1, sink);
+          sink.put(h, L.p[r], k, nn, -1);
           h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
-          emit(h, L.p[r], k, nn, +1, sink);
+          sink.put(h, L.p[r], k, nn, +1);
         } else {
           const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
-          emit(h, L.p[r], k, nn, SIGN, sink);
+          sink.put(h, L.p[r], k, nn, SIGN);
         }
       }
     }
@@ -170,12 +209,12 @@ __device__ __forceinline__ void recheck(const LanePrimes<R> &L, const LaneStates
         uint32_t nn = 0;
         if (SIGN == 0) {
           bool h = L.valid[r] && exact(Kc, n, a, k, nn);  // Kc is 2^-n: the -1 form
-          emit(h, L.p[r], k, nn, -1, sink);
+          sink.put(h, L.p[r], k, nn, -1);
           h = L.valid[r] && exact(L.p[r] - Kc, n, a, k, nn);
-          emit(h, L.p[r], k, nn, +1, sink);
+          sink.put(h, L.p[r], k, nn, +1);
         } else {
           const bool h = L.valid[r] && exact(Kc, n, a, k, nn);
-          emit(h, L.p[r], k, nn, SIGN, sink);
+          sink.put(h, L.p[r], k, nn, SIGN);
         }
       }
     }
