@@ -43,23 +43,29 @@ std::vector<uint32_t> host_primes(uint32_t lim) {
 std::vector<SmallPrime> small_prime_table() {
   std::vector<SmallPrime> sm;
   const uint64_t two32 = (uint64_t)1 << 32;
-  for (uint32_t q : host_primes(32767))
-    if (q >= 11) sm.push_back({q, (uint32_t)(two32 / q), (uint32_t)(two32 / (q - 1)), (uint32_t)(two32 % q)});
+  for (uint32_t q : host_primes(32767)) {
+    if (q < 11) continue;
+    uint32_t inv = q;  // Newton: q*q == 1 (mod 8), each step doubles the correct bits
+    for (int i = 0; i < 4; ++i) inv *= 2u - q * inv;
+    sm.push_back({q, (uint32_t)(two32 / q), (uint32_t)(two32 / (q - 1)), (uint32_t)(two32 % q), inv,
+                  (uint32_t)((two32 - 1) / q), {0, 0}});
+  }
   while (sm.size() % 64) sm.push_back(sm.back());
   return sm;
 }
 
-// Powers of two modulo each prime of that table, in the layout divides_tab
-// reads (kPowPer u16 per prime): 2^j mod q for j < kPowLo, then
-// 2^(kPowLo*j) mod q for j < kPowHi.
+// Negative powers of two modulo each prime of that table, in the layout
+// divides_tab reads (kPowPer u16 per prime): 2^-j mod q for j < kPowLo,
+// then 2^(-kPowLo*j) mod q for j < kPowHi.
 std::vector<uint16_t> pow2_table(const std::vector<SmallPrime> &sm) {
   std::vector<uint16_t> t(sm.size() * kPowPer);
   for (size_t i = 0; i < sm.size(); ++i) {
     const uint32_t q = sm[i].q;
+    const uint32_t half = (q + 1) / 2;  // 2^-1 mod q
     uint16_t *row = &t[i * kPowPer];
     uint32_t x = 1;
-    for (uint32_t j = 0; j < kPowLo; ++j, x = 2 * x % q) row[j] = (uint16_t)x;
-    const uint32_t big = x;  // 2^kPowLo mod q
+    for (uint32_t j = 0; j < kPowLo; ++j, x = x * half % q) row[j] = (uint16_t)x;
+    const uint32_t big = x;  // 2^-kPowLo mod q
     x = 1;
     for (uint32_t j = 0; j < kPowHi; ++j, x = x * big % q) row[kPowLo + j] = (uint16_t)x;
   }

@@ -17,29 +17,57 @@ __device__ __forceinline__ uint32_t mod_barrett(uint32_t x, uint32_t d, uint32_t
   return min(r, r - d);
 }
 
-// Powers of two modulo each small prime q of the filter's table, u16 each,
-// kPowPer per prime in the table's order: 2^j mod q for j < kPowLo, then
-// 2^(kPowLo*j) mod q for j < kPowHi.  Since e = n mod (q-1) < 2^15 =
-// kPowLo*kPowHi, 2^e mod q is the product of one entry of each half.
+// Negative powers of two modulo each small prime q of the filter's table,
+// u16 each, kPowPer per prime in the table's order: 2^-j mod q for j <
+// kPowLo, then 2^(-kPowLo*j) mod q for j < kPowHi.  Since e = n mod (q-1) <
+// 2^15 = kPowLo*kPowHi, 2^-e mod q is the product of one entry of each half.
 constexpr uint32_t kPowLoBits = 7;
 constexpr uint32_t kPowLo = 1u << kPowLoBits, kPowHi = 256, kPowPer = kPowLo + kPowHi;
 constexpr uint32_t kPowPassBytes = 64 * kPowPer * sizeof(uint16_t);  // one pass of 64 primes: 48 KiB
 
-// q | k*2^n + c ?  (q an odd prime below 2^15.)  k mod q, from both 32-bit
-// halves only when BIGK (the high half times 2^32 mod q); 2^n mod q as 2^e
-// with e = n mod (q-1) (Fermat), from the two halves of q's powers (row, in
-// LDS): every product is below q^2 < 2^30.  wantv is the residue k*2^n must
-// have: q - 1 for c = +1, 1 for c = -1.  About 30 VALU instructions, where
-// eleven square-and-maybe-double steps for 2^e took about 125.
-template <bool BIGK>
-__device__ __forceinline__ bool divides_tab(uint32_t klo, uint32_t khi, uint32_t n, uint32_t wantv,
-                                            const SmallPrime &sp, const uint16_t *row) {
-  const uint32_t q = sp.q;
-  uint32_t km = mod_barrett(klo, q, sp.mq);
-  if (BIGK) km = mod_barrett(mod_barrett(khi, q, sp.mq) * sp.c32 + km, q, sp.mq);
-  const uint32_t e = mod_barrett(n, q - 1, sp.md);
-  const uint32_t t = mod_barrett((uint32_t)row[e & (kPowLo - 1)] * row[kPowLo + (e >> kPowLoBits)], q, sp.mq);
-  return mod_barrett(km * t, q, sp.mq) == wantv;
+// q | k*2^n + c ?  (q an odd prime below 2^15, c = +1, or -1 for NEG.)
+// Since 2 is a unit mod q, that is q | k + c*2^-n, and 2^-n = 2^-e with
+// e = n mod (q-1) (Fermat) = lo*hi mod q for one entry of each half of q's
+// row (in LDS).  kq is k itself when k < 2^31, else k mod q from both
+// 32-bit halves (the high half times 2^32 mod q).  x = kq + lo*hi (c = +1)
+// or kq + lo*(q - hi) (c = -1, since q - hi == -hi) is congruent to
+// k + c*2^-n and below 2^31 + 2^30, so the answer is whether q | x: one
+// multiply by q^-1 mod 2^32 and one compare.  About 17 VALU instructions
+// per (hit, q).
+__device__ __forceinline__ uint32_t k_mod_q(uint32_t klo, uint32_t khi, const SmallPrime &sp) {
+  const uint32_t a = mod_barrett(khi, sp.q, sp.mq) * sp.c32 + mod_barrett(klo, sp.q, sp.mq);  // < 2^31
+  return mod_barrett(a, sp.q, sp.mq);
+}
+template <bool NEG>
+__device__ __forceinline__ bool divides_tab(uint32_t kq, uint32_t n, const SmallPrime &sp, const uint16_t *row) {
+  const uint32_t e = mod_barrett(n, sp.q - 1, sp.md);
+  const uint32_t lo = row[e & (kPowLo - 1)], hi = row[kPowLo + (e >> kPowLoBits)];
+  const uint32_t x = kq + lo * (NEG ? sp.q - hi : hi);
+  return x * sp.qinv <= sp.qlim;
+}
+
+// Of the live hits in `mask` (wave-uniform; those in `big` have k >= 2^31),
+// the ones the lane's prime q divides, for one sign: each hit in turn is
+// read into scalar registers and tested against the wave's 64 primes.
+template <bool NEG>
+__device__ __forceinline__ uint64_t divisible_hits(uint64_t mask, uint64_t big, uint64_t k, uint32_t n,
+                                                   const SmallPrime &sp, const uint16_t *row) {
+  uint64_t dead = 0;
+  while (mask) {
+    const int j = __ffsll((unsigned long long)mask) - 1;
+    mask &= mask - 1;
+    const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j);
+    const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)n, j);
+    bool dv;
+    if ((big >> j) & 1) {
+      const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(k >> 32), j);
+      dv = divides_tab<NEG>(k_mod_q(klo, khi, sp), nj, sp, row);
+    } else {
+      dv = divides_tab<NEG>(klo, nj, sp, row);
+    }
+    if (__ballot(dv)) dead |= 1ull << j;
+  }
+  return dead;
 }
 
 // The residue pre-tests of test_factor in wrapping 64-bit arithmetic with their
@@ -125,7 +153,9 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
   const uint16_t *row = lds + lane * kPowPer;  // this lane's prime's powers, in every pass
   for (uint64_t base = blockIdx.x * per; base < nraw; base += (uint64_t)gridDim.x * per) {  // workgroup-uniform
     HitT x[kFilterSlots];
-    bool alive[kFilterSlots];
+    // per slot, wave-uniform: the lanes whose hit is still alive, those
+    // whose k needs reducing mod q (k >= 2^31), those with c = -1
+    uint64_t live[kFilterSlots], big[kFilterSlots], neg[kFilterSlots];
 #pragma unroll
     for (int s = 0; s < kFilterSlots; ++s) {
       const uint64_t h = base + first + 64 * s;
@@ -134,7 +164,9 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
       if (valid) x[s] = hits[h];
       uint32_t kstep = 2, koffset = 1, nmin = 0, nmax = 0;
       if (valid) rule(x[s], kstep, koffset, nmin, nmax);
-      alive[s] = valid && rule_cheap(x[s].k, x[s].n, x[s].c, kstep, koffset, nmin, nmax);
+      live[s] = __ballot(valid && rule_cheap(x[s].k, x[s].n, x[s].c, kstep, koffset, nmin, nmax));
+      big[s] = __ballot(x[s].k >= 0x80000000ull);
+      neg[s] = __ballot(x[s].c < 0);
     }
     for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
       __syncthreads();  // the previous pass's table is no longer read
@@ -142,33 +174,21 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
       uint4 *dst = (uint4 *)lds;
       for (uint32_t i = threadIdx.x; i < kPowPassBytes / 16; i += blockDim.x) dst[i] = src[i];
       const SmallPrime sp = small_primes[j0 + lane];
-      bool any = false;
+      uint64_t any = 0;
 #pragma unroll
-      for (int s = 0; s < kFilterSlots; ++s) any |= alive[s];
-      if (!__syncthreads_or(any)) break;  // also the barrier after staging
+      for (int s = 0; s < kFilterSlots; ++s) any |= live[s];
+      if (!__syncthreads_or(any != 0)) break;  // also the barrier after staging
 #pragma unroll
       for (int s = 0; s < kFilterSlots; ++s) {
-        uint64_t mask = __ballot(alive[s]);
-        while (mask) {
-          const int j = __ffsll((unsigned long long)mask) - 1;
-          mask &= mask - 1;
-          // the hit of lane j, read into scalar registers (j is wave-uniform)
-          const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(x[s].k >> 32), j);
-          const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x[s].k, j);
-          const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)x[s].n, j);
-          const int32_t c = __builtin_amdgcn_readlane((int)x[s].c, j);
-          const uint32_t wantv = c > 0 ? sp.q - 1 : 1u;
-          const bool dv = khi ? divides_tab<true>(klo, khi, n, wantv, sp, row)
-                              : divides_tab<false>(klo, khi, n, wantv, sp, row);
-          if (__ballot(dv) != 0 && (int)lane == j) alive[s] = false;
-        }
+        const uint64_t dead = divisible_hits<false>(live[s] & ~neg[s], big[s], x[s].k, x[s].n, sp, row) |
+                              divisible_hits<true>(live[s] & neg[s], big[s], x[s].k, x[s].n, sp, row);
+        live[s] &= ~dead;
       }
     }
 #pragma unroll
     for (int s = 0; s < kFilterSlots; ++s) {
-      const uint64_t keepmask = __ballot(alive[s]);
-      const unsigned long long idx = wave_slot(keepmask, kept_count);
-      if (alive[s] && idx < kept_cap) kept[idx] = x[s];
+      const unsigned long long idx = wave_slot(live[s], kept_count);
+      if (((live[s] >> lane) & 1) && idx < kept_cap) kept[idx] = x[s];
     }
   }
 }

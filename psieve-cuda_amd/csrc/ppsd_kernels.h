@@ -6,18 +6,21 @@
 
 namespace ppsd {
 
-// One raw hit: p | k*2^n + c, with k odd in [kmin,kmax] and nmin <= n < nmax.
-// Same field order and widths as ppsd_factor in include/ppsd.h.
 // The constants of one small prime's arithmetic for the factor filter,
 // computed on the host.  The device table is padded to whole passes of 64
 // with copies of its last prime (a repeated divisor changes no verdict).
 struct SmallPrime {
   uint32_t q;
-  uint32_t mq;   // floor(2^32 / q)
-  uint32_t md;   // floor(2^32 / (q - 1))
-  uint32_t c32;  // 2^32 mod q
+  uint32_t mq;    // floor(2^32 / q)
+  uint32_t md;    // floor(2^32 / (q - 1))
+  uint32_t c32;   // 2^32 mod q
+  uint32_t qinv;  // q^-1 mod 2^32
+  uint32_t qlim;  // floor((2^32 - 1) / q): for x < 2^32, q | x iff x * qinv mod 2^32 <= qlim
+  uint32_t pad[2];
 };
 
+// One raw hit: p | k*2^n + c, with k odd in [kmin,kmax] and nmin <= n < nmax.
+// Same field order and widths as ppsd_factor in include/ppsd.h.
 struct Hit {
   uint64_t p;
   uint64_t k;
