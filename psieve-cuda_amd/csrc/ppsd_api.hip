@@ -1,6 +1,8 @@
 // ppsd_api.hip -- host half of libppsd.so: the C ABI declared in
 // include/ppsd.h.  Owns every device buffer of a context; launches the
 // kernels of ppsd_kernels.hip on one non-blocking stream per context.
+#include <thread>
+
 #include "ppsd_ctx.h"
 
 using namespace ppsd;
@@ -179,6 +181,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   (void)hipFree(c->d_P);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_counters);
@@ -227,6 +230,9 @@ static int free_ctx(ppsd_ctx *c) {
   if (c->h_tasks) (void)hipHostFree(c->h_tasks);
   if (c->h_ntasks) (void)hipHostFree(c->h_ntasks);
   if (c->h_bstage) (void)hipHostFree(c->h_bstage);
+  if (c->h_out) (void)hipHostFree(c->h_out);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+  if (c->ev_emit) (void)hipEventDestroy(c->ev_emit);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -846,6 +852,10 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipMalloc(&c->b_out, c->b_out_cap * sizeof(BHit)));
   c->h_bstage_cap = (uint64_t)1 << 20;
   HIP_TRY(hipHostMalloc((void **)&c->h_bstage, c->h_bstage_cap * sizeof(BHit), hipHostMallocDefault));
+  c->h_out_cap = (uint64_t)1 << 24;  // 512 MiB
+  HIP_TRY(hipHostMalloc((void **)&c->h_out, c->h_out_cap * sizeof(BHit), hipHostMallocDefault));
+  HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&c->ev_emit, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
@@ -1003,10 +1013,15 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
 // the raw ring overflows.  Appends kept (or raw) hits to b_result.
 int ppsd::batch_flush(ppsd_ctx *c) {
   HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamSynchronize(c->copy_stream));
   if (c->b_out_n) {
     const size_t old = c->b_result.size();
+    const uint64_t mirrored = std::min(c->b_out_n, c->h_out_cap);
     c->b_result.resize(old + c->b_out_n);
-    HIP_TRY(hipMemcpy(c->b_result.data() + old, c->b_out, c->b_out_n * sizeof(BHit), hipMemcpyDeviceToHost));
+    memcpy(c->b_result.data() + old, c->h_out, mirrored * sizeof(BHit));
+    if (c->b_out_n > mirrored)
+      HIP_TRY(hipMemcpy(c->b_result.data() + old + mirrored, c->b_out + mirrored,
+                        (c->b_out_n - mirrored) * sizeof(BHit), hipMemcpyDeviceToHost));
     c->b_out_n = 0;
   }
   return 0;
@@ -1052,6 +1067,13 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int b
       if (frc) return frc;
     }
     HIP_TRY(launch_emit(raw ? c->b_hits : c->b_kept, take, c->b_pieces, c->b_out + c->b_out_n, c->stream));
+    if (take && c->b_out_n < c->h_out_cap) {  // to the host mirror beside the next walk
+      const uint64_t m = std::min(take, c->h_out_cap - c->b_out_n);
+      HIP_TRY(hipEventRecord(c->ev_emit, c->stream));
+      HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->ev_emit, 0));
+      HIP_TRY(hipMemcpyAsync(c->h_out + c->b_out_n, c->b_out + c->b_out_n, m * sizeof(BHit),
+                             hipMemcpyDeviceToHost, c->copy_stream));
+    }
     c->b_out_n += take;
     c->st.raw_hits += cnt[0];
     c->st.factors += cnt[1];
@@ -1245,6 +1267,7 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   int rc = batch_alloc(c);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream2));  // nothing of an earlier, failed run still in flight
+  HIP_TRY(hipStreamSynchronize(c->copy_stream));
   if (nunits) {
     rc = ppsd_batch_prepare(c, pmax_all);
     if (rc) return rc;
@@ -1316,23 +1339,45 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
   return 0;
 }
 
+// Records [0, n) of src into out / unit_of, split over a few host threads
+// (tens of millions of records for a hit-dense batch).
+static void put_results(const BHit *src, uint64_t n, ppsd_factor *out, uint32_t *unit_of) {
+  auto part = [=](uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) {
+      const BHit &h = src[i];
+      if (out) out[i] = ppsd_factor{h.p, h.k, h.n, h.c};
+      if (unit_of) unit_of[i] = h.piece;
+    }
+  };
+  const uint64_t per_thread = (uint64_t)1 << 20;
+  const unsigned nt = (unsigned)std::min<uint64_t>(8, (n + per_thread - 1) / per_thread);
+  if (nt <= 1) {
+    part(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t) th.emplace_back(part, n * t / nt, n * (t + 1) / nt);
+  for (auto &x : th) x.join();
+}
+
 int ppsd_batch_results(ppsd_ctx *c, ppsd_factor *out, uint32_t *unit_of, uint64_t cap, uint64_t *n_out) {
   if (!c) return set_err(PPSD_ERR_INVALID_PARAM, "null context");
   const uint64_t nhost = c->b_result.size();
   if (n_out) *n_out = nhost + c->b_out_n;
   const uint64_t n = std::min<uint64_t>(cap, nhost + c->b_out_n);
-  auto put = [&](uint64_t i, const BHit &h) {
-    if (out) out[i] = ppsd_factor{h.p, h.k, h.n, h.c};
-    if (unit_of) unit_of[i] = h.piece;
-  };
-  for (uint64_t i = 0; i < std::min(n, nhost); ++i) put(i, c->b_result[i]);
   if (!out && !unit_of) return 0;
-  // the device part, through the pinned staging buffer
-  for (uint64_t i = nhost; i < n;) {
+  put_results(c->b_result.data(), std::min(n, nhost), out, unit_of);
+  if (n <= nhost) return 0;
+  // the device part: its mirrored head from pinned memory, the rest through
+  // the pinned staging buffer
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->copy_stream));
+  const uint64_t mirrored = std::min(n - nhost, c->h_out_cap);
+  put_results(c->h_out, mirrored, out ? out + nhost : nullptr, unit_of ? unit_of + nhost : nullptr);
+  for (uint64_t i = nhost + mirrored; i < n;) {
     const uint64_t m = std::min(n - i, c->h_bstage_cap);
-    HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpy(c->h_bstage, c->b_out + (i - nhost), m * sizeof(BHit), hipMemcpyDeviceToHost));
-    for (uint64_t j = 0; j < m; ++j) put(i + j, c->h_bstage[j]);
+    put_results(c->h_bstage, m, out ? out + i : nullptr, unit_of ? unit_of + i : nullptr);
     i += m;
   }
   return 0;

@@ -158,6 +158,14 @@ struct ppsd_ctx {
   std::vector<BHit, ppsd::NoInitAlloc<BHit>> b_result;  // results copied back (batch_flush)
   BHit *h_bstage = nullptr;  // pinned: b_out read back in pieces by ppsd_batch_results
   uint64_t h_bstage_cap = 0;
+  // pinned mirror of b_out[0, h_out_cap): each task group's output is copied
+  // on copy_stream as soon as it is emitted, beside the next walk, so that
+  // the results are on the host when the batch ends (valid once copy_stream
+  // has finished)
+  BHit *h_out = nullptr;
+  uint64_t h_out_cap = 0;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_emit = nullptr;
   std::vector<Task> b_task_list;  // a round's tasks, built on the host (capacity kept)
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;

@@ -47,25 +47,38 @@ __device__ __forceinline__ bool divides_tab(uint32_t kq, uint32_t n, const Small
 }
 
 // Of the live hits in `mask` (wave-uniform; those in `big` have k >= 2^31),
-// the ones the lane's prime q divides, for one sign: each hit in turn is
-// read into scalar registers and tested against the wave's 64 primes.
+// the ones the lane's prime q divides, for one sign: each hit is read into
+// scalar registers and tested against the wave's 64 primes.  Hits with
+// k < 2^31 go two at a time (with an odd count the last one twice), so that
+// two independent chains of multiplies and LDS reads interleave.
 template <bool NEG>
 __device__ __forceinline__ uint64_t divisible_hits(uint64_t mask, uint64_t big, uint64_t k, uint32_t n,
                                                    const SmallPrime &sp, const uint16_t *row) {
   uint64_t dead = 0;
-  while (mask) {
-    const int j = __ffsll((unsigned long long)mask) - 1;
-    mask &= mask - 1;
-    const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j);
-    const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)n, j);
-    bool dv;
-    if ((big >> j) & 1) {
-      const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(k >> 32), j);
-      dv = divides_tab<NEG>(k_mod_q(klo, khi, sp), nj, sp, row);
-    } else {
-      dv = divides_tab<NEG>(klo, nj, sp, row);
+  for (uint64_t m = mask & ~big; m;) {
+    const int j1 = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1;
+    int j2 = j1;
+    if (m) {
+      j2 = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
     }
-    if (__ballot(dv)) dead |= 1ull << j;
+    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j1);
+    const uint32_t n1 = (uint32_t)__builtin_amdgcn_readlane((int)n, j1);
+    const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j2);
+    const uint32_t n2 = (uint32_t)__builtin_amdgcn_readlane((int)n, j2);
+    const bool d1 = divides_tab<NEG>(k1, n1, sp, row);
+    const bool d2 = divides_tab<NEG>(k2, n2, sp, row);
+    if (__ballot(d1)) dead |= 1ull << j1;
+    if (__ballot(d2)) dead |= 1ull << j2;
+  }
+  for (uint64_t m = mask & big; m;) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1;
+    const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j);
+    const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(k >> 32), j);
+    const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)n, j);
+    if (__ballot(divides_tab<NEG>(k_mod_q(klo, khi, sp), nj, sp, row))) dead |= 1ull << j;
   }
   return dead;
 }

@@ -157,6 +157,7 @@ void finish_slice(ppsd_server *s, Slice &sl) {
   ppsd_ctx *c = s->ctx;
   const size_t np = sl.pieces.size();
   const double t0 = s->now();
+  (void)hipStreamSynchronize(c->copy_stream);  // no copy of an earlier, failed slice still in flight
   c->b_result.clear();
   c->b_out_n = 0;
   c->b_unit_primes.assign(np, 0);
