@@ -915,11 +915,14 @@ static BatchArgs batch_args(ppsd_ctx *c, int buf) {
   return a;
 }
 
-static uint32_t persist_grid(const ppsd_ctx *c) {
-  hipDeviceProp_t prop;
-  int cus = 256;
-  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
-  return (uint32_t)cus * (uint32_t)c->b_waves;  // 256-thread groups: b_waves waves per SIMD
+static uint32_t persist_grid(ppsd_ctx *c) {
+  if (!c->cus) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+      cus = 256;
+    c->cus = (uint32_t)cus;
+  }
+  return c->cus * (uint32_t)c->b_waves;  // 256-thread groups: b_waves waves per SIMD
 }
 
 // The round's device sequence -- zero the counters and the queue head, drain
@@ -1007,6 +1010,31 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
       }
   (void)par;
   return seg;
+}
+
+// A launch drains its queue largest task first, so its end is set by the
+// last tasks the waves take: with equal tasks of a whole segment each, waves
+// finish up to a task's length apart (a segment of 65536 windows takes a
+// wave sharing its SIMD with seven others about 10 ms).  The group's last
+// 2 tasks per resident wave are therefore cut into kTailParts shorter
+// segments (none below kTailMinWindows windows, whose start states cost
+// about 1 % of their walk) and queued after the rest, in rounds of one part
+// of each: the waves then run out of work within one short task of each
+// other.  Tasks [0, n) of `in`, in that order, go to `out`.
+constexpr uint32_t kTailParts = 8;
+constexpr uint32_t kTailMinWindows = 4096;
+static void split_tail(const Task *in, size_t n, uint64_t waves, std::vector<Task> &out) {
+  const size_t ntail = std::min<size_t>(n, 2 * waves);
+  out.assign(in, in + (n - ntail));
+  const Task *tail = in + (n - ntail);
+  for (uint32_t j = 0; j < kTailParts; ++j)
+    for (size_t i = 0; i < ntail; ++i) {
+      const Task &T = tail[i];
+      const uint32_t parts = std::max<uint32_t>(1, std::min<uint32_t>(kTailParts, T.nw / kTailMinWindows));
+      if (j >= parts) continue;
+      const uint32_t w0 = (uint32_t)((uint64_t)T.nw * j / parts), w1 = (uint32_t)((uint64_t)T.nw * (j + 1) / parts);
+      out.push_back(Task{T.piece, T.first, T.count, T.w0 + w0, w1 - w0});
+    }
 }
 
 // Run tasks [0, n) of h_tasks through the round graph; re-run halves when
@@ -1214,17 +1242,21 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     hpw[i] = D.nwin ? hits_per_prime(*par[pieces[b + i].unit], pieces[b + i].lo | 1) / D.nwin : 0.0;
   }
   const double room = 0.5 * (double)c->b_hit_cap;
+  const uint64_t waves = (uint64_t)persist_grid(c) * 4;
+  const size_t group_max = c->b_task_cap - (size_t)(2 * waves * kTailParts);  // room for the split tail
+  std::vector<Task> &grp = c->b_group;
   for (size_t t = 0; t < tasks.size();) {
     size_t k = 0;
     double e = 0;
-    while (t + k < tasks.size() && k < c->b_task_cap) {
+    while (t + k < tasks.size() && k < group_max) {
       const Task &T = tasks[t + k];
       const double et = hpw[T.piece] * T.count * T.nw;
       if (k && e + et > room) break;
       e += et;
       ++k;
     }
-    int rc = run_tasks(c, tasks.data() + t, (uint32_t)k, raw, buf);
+    split_tail(tasks.data() + t, k, waves, grp);
+    int rc = run_tasks(c, grp.data(), (uint32_t)grp.size(), raw, buf);
     if (rc) return rc;
     t += k;
   }

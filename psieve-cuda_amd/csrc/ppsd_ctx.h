@@ -122,6 +122,7 @@ struct ppsd_ctx {
 
   // batched walk (allocated on first use)
   bool b_ready = false;
+  uint32_t cus = 0;  // compute units of the device (read on first use)
   uint64_t b_pool_cap = 0;
   // two prime pools: the next round is sieved (on stream2) while the
   // current one is walked (on stream)
@@ -167,6 +168,7 @@ struct ppsd_ctx {
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_emit = nullptr;
   std::vector<Task> b_task_list;  // a round's tasks, built on the host (capacity kept)
+  std::vector<Task> b_group;      // one launch's tasks, its tail cut shorter (capacity kept)
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;
 };
