@@ -212,7 +212,8 @@ int ppsd_batch_unit_primes(const ppsd_ctx *ctx, uint64_t *count, uint64_t *sum, 
 /* Shape of the batched kernel: r primes per lane (1, 2), u windows per
  * pre-test chunk (4, 8, 16), resident waves per SIMD (1..16), and the
  * n-segment length in windows, chosen per run between seg_min and seg_max
- * (seg_min >= 16).  0 keeps a value. */
+ * (seg_min >= 16).  0 keeps a value.  Defaults: r = 1, u = 16, 6 waves per
+ * SIMD, segments of 2048..65536 windows. */
 int ppsd_batch_set_tuning(ppsd_ctx *ctx, int r, int u, int waves_per_simd, uint32_t seg_min, uint32_t seg_max);
 int ppsd_batch_get_tuning(const ppsd_ctx *ctx, int *r, int *u, int *waves_per_simd, uint32_t *seg_min,
                           uint32_t *seg_max);

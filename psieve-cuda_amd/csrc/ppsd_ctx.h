@@ -152,7 +152,7 @@ struct ppsd_ctx {
   hipGraph_t b_graph[2] = {};           // per pool: the round graph
   hipGraphExec_t b_exec[2] = {};
   int b_graph_key[2] = {-1, -1};
-  int b_r = 2, b_u = 16, b_waves = 8;
+  int b_r = 1, b_u = 16, b_waves = 6;  // defaults: the tuner's choice on MI355X
   uint32_t b_seg_min = 2048, b_seg_max = 65536;
   BHit *b_out = nullptr;  // the batch's results so far, on the device, unit-indexed
   uint64_t b_out_cap = 0, b_out_n = 0;

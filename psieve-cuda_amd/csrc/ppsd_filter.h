@@ -49,28 +49,34 @@ __device__ __forceinline__ bool divides_tab(uint32_t kq, uint32_t n, const Small
 // Of the live hits in `mask` (wave-uniform; those in `big` have k >= 2^31),
 // the ones the lane's prime q divides, for one sign: each hit is read into
 // scalar registers and tested against the wave's 64 primes.  Hits with
-// k < 2^31 go two at a time (with an odd count the last one twice), so that
-// two independent chains of multiplies and LDS reads interleave.
+// k < 2^31 go kHitsPerTurn at a time (the last turn repeats its last hit to
+// fill up), so that independent chains of multiplies and LDS reads
+// interleave.
+constexpr int kHitsPerTurn = 2;
 template <bool NEG>
 __device__ __forceinline__ uint64_t divisible_hits(uint64_t mask, uint64_t big, uint64_t k, uint32_t n,
                                                    const SmallPrime &sp, const uint16_t *row) {
   uint64_t dead = 0;
   for (uint64_t m = mask & ~big; m;) {
-    const int j1 = __ffsll((unsigned long long)m) - 1;
-    m &= m - 1;
-    int j2 = j1;
-    if (m) {
-      j2 = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
+    int j[kHitsPerTurn];
+#pragma unroll
+    for (int h = 0; h < kHitsPerTurn; ++h) {
+      j[h] = h ? j[h - 1] : 0;
+      if (m) {
+        j[h] = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+      }
     }
-    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j1);
-    const uint32_t n1 = (uint32_t)__builtin_amdgcn_readlane((int)n, j1);
-    const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j2);
-    const uint32_t n2 = (uint32_t)__builtin_amdgcn_readlane((int)n, j2);
-    const bool d1 = divides_tab<NEG>(k1, n1, sp, row);
-    const bool d2 = divides_tab<NEG>(k2, n2, sp, row);
-    if (__ballot(d1)) dead |= 1ull << j1;
-    if (__ballot(d2)) dead |= 1ull << j2;
+    bool d[kHitsPerTurn];
+#pragma unroll
+    for (int h = 0; h < kHitsPerTurn; ++h) {
+      const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, j[h]);
+      const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)n, j[h]);
+      d[h] = divides_tab<NEG>(kj, nj, sp, row);
+    }
+#pragma unroll
+    for (int h = 0; h < kHitsPerTurn; ++h)
+      if (__ballot(d[h])) dead |= 1ull << j[h];
   }
   for (uint64_t m = mask & big; m;) {
     const int j = __ffsll((unsigned long long)m) - 1;
@@ -166,9 +172,7 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
   const uint16_t *row = lds + lane * kPowPer;  // this lane's prime's powers, in every pass
   for (uint64_t base = blockIdx.x * per; base < nraw; base += (uint64_t)gridDim.x * per) {  // workgroup-uniform
     HitT x[kFilterSlots];
-    // per slot, wave-uniform: the lanes whose hit is still alive, those
-    // whose k needs reducing mod q (k >= 2^31), those with c = -1
-    uint64_t live[kFilterSlots], big[kFilterSlots], neg[kFilterSlots];
+    bool alive[kFilterSlots];
 #pragma unroll
     for (int s = 0; s < kFilterSlots; ++s) {
       const uint64_t h = base + first + 64 * s;
@@ -177,9 +181,7 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
       if (valid) x[s] = hits[h];
       uint32_t kstep = 2, koffset = 1, nmin = 0, nmax = 0;
       if (valid) rule(x[s], kstep, koffset, nmin, nmax);
-      live[s] = __ballot(valid && rule_cheap(x[s].k, x[s].n, x[s].c, kstep, koffset, nmin, nmax));
-      big[s] = __ballot(x[s].k >= 0x80000000ull);
-      neg[s] = __ballot(x[s].c < 0);
+      alive[s] = valid && rule_cheap(x[s].k, x[s].n, x[s].c, kstep, koffset, nmin, nmax);
     }
     for (uint32_t j0 = 0; j0 < nsmall; j0 += 64) {
       __syncthreads();  // the previous pass's table is no longer read
@@ -187,21 +189,29 @@ __device__ __forceinline__ void filter_hits(const HitT *hits, uint64_t nraw, Hit
       uint4 *dst = (uint4 *)lds;
       for (uint32_t i = threadIdx.x; i < kPowPassBytes / 16; i += blockDim.x) dst[i] = src[i];
       const SmallPrime sp = small_primes[j0 + lane];
-      uint64_t any = 0;
+      bool any = false;
 #pragma unroll
-      for (int s = 0; s < kFilterSlots; ++s) any |= live[s];
-      if (!__syncthreads_or(any != 0)) break;  // also the barrier after staging
+      for (int s = 0; s < kFilterSlots; ++s) any |= alive[s];
+      if (!__syncthreads_or(any)) break;  // also the barrier after staging
 #pragma unroll
       for (int s = 0; s < kFilterSlots; ++s) {
-        const uint64_t dead = divisible_hits<false>(live[s] & ~neg[s], big[s], x[s].k, x[s].n, sp, row) |
-                              divisible_hits<true>(live[s] & neg[s], big[s], x[s].k, x[s].n, sp, row);
-        live[s] &= ~dead;
+        // wave-uniform masks of the slot: live hits, those with k >= 2^31,
+        // those with c = -1 (ballots once per pass, so that only the live
+        // mask's walk holds scalar registers through the hit loop)
+        const uint64_t live = __ballot(alive[s]);
+        if (!live) continue;
+        const uint64_t big = __ballot(x[s].k >= 0x80000000ull);
+        const uint64_t neg = __ballot(x[s].c < 0);
+        const uint64_t dead = divisible_hits<false>(live & ~neg, big, x[s].k, x[s].n, sp, row) |
+                              divisible_hits<true>(live & neg, big, x[s].k, x[s].n, sp, row);
+        if ((dead >> lane) & 1) alive[s] = false;
       }
     }
 #pragma unroll
     for (int s = 0; s < kFilterSlots; ++s) {
-      const unsigned long long idx = wave_slot(live[s], kept_count);
-      if (((live[s] >> lane) & 1) && idx < kept_cap) kept[idx] = x[s];
+      const uint64_t keep = __ballot(alive[s]);
+      const unsigned long long idx = wave_slot(keep, kept_count);
+      if (alive[s] && idx < kept_cap) kept[idx] = x[s];
     }
   }
 }
