@@ -218,6 +218,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->b_pieces);
   (void)hipFree(c->b_tasks);
   (void)hipFree(c->b_ntasks);
+  (void)hipFree(c->b_dclasses);
   (void)hipFree(c->b_next);
   (void)hipFree(c->b_hits);
   (void)hipFree(c->b_kept);
@@ -227,8 +228,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->b_rblockcount);
   (void)hipFree(c->b_rblocksum);
   if (c->h_pieces) (void)hipHostFree(c->h_pieces);
-  if (c->h_tasks) (void)hipHostFree(c->h_tasks);
-  if (c->h_ntasks) (void)hipHostFree(c->h_ntasks);
+  if (c->h_classes) (void)hipHostFree(c->h_classes);
   if (c->h_bstage) (void)hipHostFree(c->h_bstage);
   if (c->h_out) (void)hipHostFree(c->h_out);
   if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
@@ -842,6 +842,9 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipMalloc(&c->b_pieces, c->b_piece_cap * sizeof(PieceDesc)));
   HIP_TRY(hipMalloc(&c->b_tasks, (size_t)c->b_task_cap * sizeof(Task)));
   HIP_TRY(hipMalloc(&c->b_ntasks, sizeof(uint32_t)));
+  c->b_class_cap = 4 * c->b_piece_cap;  // at most four classes a piece
+  HIP_TRY(hipMalloc(&c->b_dclasses, (size_t)c->b_class_cap * sizeof(TaskClass)));
+  HIP_TRY(hipHostMalloc((void **)&c->h_classes, (size_t)c->b_class_cap * sizeof(TaskClass), hipHostMallocDefault));
   HIP_TRY(hipMalloc(&c->b_next, sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&c->b_hits, c->b_hit_alloc * sizeof(BHit)));
   HIP_TRY(hipMalloc(&c->b_kept, c->b_hit_alloc * sizeof(BHit)));
@@ -857,8 +860,6 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&c->ev_emit, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc((void **)&c->h_pieces, c->b_piece_cap * sizeof(PieceDesc), hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void **)&c->h_tasks, (size_t)c->b_task_cap * sizeof(Task), hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc((void **)&c->h_ntasks, sizeof(uint32_t), hipHostMallocDefault));
   c->b_ready = true;
   return 0;
 }
@@ -952,11 +953,15 @@ static int round_graph(ppsd_ctx *c, int buf) {
   return 0;
 }
 
-// Tasks of one round over the sieved pieces; the segment length halves from
-// b_seg_max down to b_seg_min until the queue holds enough tasks to keep
-// every wave slot busy through the tail; largest tasks first.
-static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> &par, uint32_t npieces,
-                            std::vector<Task> &tasks) {
+// Task classes of one round over the sieved pieces (c->b_classes): the
+// segment length halves from b_seg_max down to b_seg_min until the queue
+// holds enough tasks to keep every wave slot busy through the tail.  A
+// piece's tasks come in at most four sizes -- full or last block of primes
+// times full or last segment of windows -- so the round's order, largest
+// task first, is its classes sorted by task size; the tasks themselves are
+// written on the device (launch_expand), which spares the host millions of
+// task records a round.  Returns the number of tasks.
+static uint64_t build_classes(ppsd_ctx *c, uint32_t npieces) {
   const uint32_t per = 64u * (uint32_t)c->b_r;
   const uint64_t target = (uint64_t)persist_grid(c) * 4 * 8;  // waves resident x 8
   uint32_t seg = c->b_seg_max;
@@ -969,76 +974,46 @@ static uint32_t build_tasks(ppsd_ctx *c, const std::vector<const ppsd_params *> 
     if (n >= target || seg <= c->b_seg_min) break;
     seg /= 2;
   }
-  // Largest first.  A piece's tasks come in at most four sizes: full or last
-  // block of primes times full or last segment of windows.  Ordering those
-  // classes (a few per piece) and emitting each class's tasks in turn gives
-  // the size order without sorting the tasks themselves (tens of thousands
-  // a round, which a comparison sort took milliseconds of host time over).
-  struct Class {
-    uint64_t size;
-    uint32_t piece, f0, f1, fstep, w0, w1, wstep;
-  };
-  std::vector<Class> cls;
-  cls.reserve(4 * (size_t)npieces);
-  uint64_t total = 0;
+  std::vector<TaskClass> &cls = c->b_classes;
+  cls.clear();
   for (uint32_t i = 0; i < npieces; ++i) {
     const PieceDesc &P = c->h_pieces[i];
     if (!P.nprimes || !P.nwin) continue;
     const uint32_t nfull = P.nprimes / per, flast = P.nprimes % per;  // full blocks, last block's count
     const uint32_t wfull = P.nwin / seg, wlast = P.nwin % seg;
-    const uint32_t fend = nfull * per, wend = wfull * seg;
-    // [f0, f1) x [w0, w1) in steps of per and seg: every task of the class
-    // has `count` primes and `nw` windows
-    auto add = [&](uint32_t f0, uint32_t f1, uint32_t count, uint32_t w0, uint32_t w1, uint32_t nw) {
-      if (f0 >= f1 || w0 >= w1) return;
-      cls.push_back(Class{(uint64_t)count * nw, i, f0, f1, per, w0, w1, seg});
-      total += (uint64_t)((f1 - f0 + per - 1) / per) * ((w1 - w0 + seg - 1) / seg);
+    // nf blocks of `count` primes from f0, nseg segments of nw windows from w0
+    auto add = [&](uint32_t f0, uint32_t nf, uint32_t count, uint32_t w0, uint32_t nseg, uint32_t nw) {
+      if (nf && nseg && count && nw) cls.push_back(TaskClass{0, i, f0, per, nf, count, w0, nw, nseg});
     };
-    add(0, fend, per, 0, wend, seg);
-    add(0, fend, per, wend, P.nwin, wlast);
-    add(fend, P.nprimes, flast, 0, wend, seg);
-    add(fend, P.nprimes, flast, wend, P.nwin, wlast);
+    add(0, nfull, per, 0, wfull, seg);
+    add(0, nfull, per, wfull * seg, 1, wlast);
+    add(nfull * per, 1, flast, 0, wfull, seg);
+    add(nfull * per, 1, flast, wfull * seg, 1, wlast);
   }
-  std::stable_sort(cls.begin(), cls.end(), [](const Class &x, const Class &y) { return x.size > y.size; });
-  tasks.clear();
-  tasks.reserve(total);
-  for (const Class &k : cls)
-    for (uint32_t f = k.f0; f < k.f1; f += k.fstep)
-      for (uint32_t w = k.w0; w < k.w1; w += k.wstep) {
-        const PieceDesc &P = c->h_pieces[k.piece];
-        tasks.push_back(Task{k.piece, f, std::min(per, P.nprimes - f), w, std::min(seg, P.nwin - w)});
-      }
-  (void)par;
-  return seg;
+  std::stable_sort(cls.begin(), cls.end(), [](const TaskClass &x, const TaskClass &y) {
+    return (uint64_t)x.count * x.nw > (uint64_t)y.count * y.nw;
+  });
+  uint64_t t = 0;
+  for (TaskClass &k : cls) {
+    k.t_begin = t;
+    t += (uint64_t)k.nf * k.nseg;
+  }
+  return t;
 }
 
 // A launch drains its queue largest task first, so its end is set by the
 // last tasks the waves take: with equal tasks of a whole segment each, waves
 // finish up to a task's length apart (a segment of 65536 windows takes a
-// wave sharing its SIMD with seven others about 10 ms).  The group's last
-// 2 tasks per resident wave are therefore cut into kTailParts shorter
+// wave sharing its SIMD with five others several ms).  A launch's last 2
+// tasks per resident wave are therefore cut into kTailParts shorter
 // segments (none below kTailMinWindows windows, whose start states cost
-// about 1 % of their walk) and queued after the rest, in rounds of one part
-// of each: the waves then run out of work within one short task of each
-// other.  Tasks [0, n) of `in`, in that order, go to `out`.
+// about 1 % of their walk) and queued after the rest, one part of each in
+// turn: the waves then run out of work within one short task of each other.
 constexpr uint32_t kTailParts = 8;
 constexpr uint32_t kTailMinWindows = 4096;
-static void split_tail(const Task *in, size_t n, uint64_t waves, std::vector<Task> &out) {
-  const size_t ntail = std::min<size_t>(n, 2 * waves);
-  out.assign(in, in + (n - ntail));
-  const Task *tail = in + (n - ntail);
-  for (uint32_t j = 0; j < kTailParts; ++j)
-    for (size_t i = 0; i < ntail; ++i) {
-      const Task &T = tail[i];
-      const uint32_t parts = std::max<uint32_t>(1, std::min<uint32_t>(kTailParts, T.nw / kTailMinWindows));
-      if (j >= parts) continue;
-      const uint32_t w0 = (uint32_t)((uint64_t)T.nw * j / parts), w1 = (uint32_t)((uint64_t)T.nw * (j + 1) / parts);
-      out.push_back(Task{T.piece, T.first, T.count, T.w0 + w0, w1 - w0});
-    }
-}
 
-// Run tasks [0, n) of h_tasks through the round graph; re-run halves when
-// the raw ring overflows.  Appends kept (or raw) hits to b_result.
+// Copies the batch's device output so far (b_out, mirrored in h_out) to
+// b_result and empties b_out.
 int ppsd::batch_flush(ppsd_ctx *c) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipStreamSynchronize(c->copy_stream));
@@ -1055,18 +1030,32 @@ int ppsd::batch_flush(ppsd_ctx *c) {
   return 0;
 }
 
-static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int buf) {
-  std::vector<std::pair<uint32_t, uint32_t>> todo{{0, n}};
+// Run tasks [t0, t1) of the round's order (c->b_classes, on the device in
+// b_dclasses) through the round graph, queued by launch_expand; re-run
+// halves when the raw ring overflows.  Appends kept (or raw) hits to b_out.
+static int run_tasks(ppsd_ctx *c, uint64_t t0, uint64_t t1, bool raw, int buf) {
+  const uint64_t waves = (uint64_t)persist_grid(c) * 4;
+  std::vector<std::pair<uint64_t, uint64_t>> todo{{t0, t1}};
   while (!todo.empty()) {
-    auto [f, k] = todo.back();
+    auto [f, e] = todo.back();
     todo.pop_back();
+    const uint64_t k = e - f;
     if (k == 0) continue;
     int rc = round_graph(c, buf);
     if (rc) return rc;
-    memcpy(c->h_tasks, tasks + f, (size_t)k * sizeof(Task));
-    *c->h_ntasks = k;
-    HIP_TRY(hipMemcpyAsync(c->b_tasks, c->h_tasks, (size_t)k * sizeof(Task), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->b_ntasks, c->h_ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    ExpandArgs x{};
+    x.classes = c->b_dclasses;
+    x.nclasses = (uint32_t)c->b_classes.size();
+    x.t0 = f;
+    x.t1 = e;
+    x.tail = (uint32_t)std::min<uint64_t>(k, 2 * waves);
+    x.parts = kTailParts;
+    x.tail_min_windows = kTailMinWindows;
+    x.out = c->b_tasks;
+    x.nout = c->b_ntasks;
+    if ((k - x.tail) + (uint64_t)x.tail * x.parts > c->b_task_cap)
+      return set_err(PPSD_ERR_FAILURE, "launch of %llu tasks exceeds the queue", (unsigned long long)k);
+    HIP_TRY(launch_expand(x, c->stream));
     unsigned long long cnt[2];
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     HIP_TRY(hipGraphLaunch(c->b_exec[buf], c->stream));
@@ -1082,8 +1071,8 @@ static int run_tasks(ppsd_ctx *c, const Task *tasks, uint32_t n, bool raw, int b
         return set_err(PPSD_ERR_INSUFFICIENT_RESOURCE, "one task has %llu hits (ring %llu)", cnt[0],
                        (unsigned long long)c->b_hit_cap);
       }
-      todo.push_back({f + k / 2, k - k / 2});
-      todo.push_back({f, k / 2});
+      todo.push_back({f + k / 2, e});
+      todo.push_back({f, f + k / 2});
       continue;
     }
     // the results stay on the device until the batch ends or its output
@@ -1233,34 +1222,49 @@ int ppsd::walk_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, si
     c->st.window_steps += (uint64_t)D.nprimes * D.nwin * (D.sign == 0 ? 2 : 1);
   }
   HIP_TRY(hipMemcpyAsync(c->b_pieces, c->h_pieces, np * sizeof(PieceDesc), hipMemcpyHostToDevice, c->stream));
-  std::vector<Task> &tasks = c->b_task_list;  // its capacity is kept from round to round
-  build_tasks(c, par, np, tasks);
-  // task groups whose expected raw hits fill at most half the ring
-  std::vector<double> hpw(np);  // expected hits per prime-window of each piece
-  for (uint32_t i = 0; i < np; ++i) {
-    const PieceDesc &D = c->h_pieces[i];
-    hpw[i] = D.nwin ? hits_per_prime(*par[pieces[b + i].unit], pieces[b + i].lo | 1) / D.nwin : 0.0;
-  }
+  const uint64_t ntask = build_classes(c, np);
+  if (!ntask) return 0;
+  const std::vector<TaskClass> &cls = c->b_classes;
+  if (cls.size() > c->b_class_cap)
+    return set_err(PPSD_ERR_FAILURE, "%zu task classes in one round (at most %u)", cls.size(), c->b_class_cap);
+  memcpy(c->h_classes, cls.data(), cls.size() * sizeof(TaskClass));
+  HIP_TRY(hipMemcpyAsync(c->b_dclasses, c->h_classes, cls.size() * sizeof(TaskClass), hipMemcpyHostToDevice,
+                         c->stream));
+  // launches whose expected raw hits fill at most half the ring: every task
+  // of a class expects the same number, so each boundary falls at a task
+  // index computed per class
   const double room = 0.5 * (double)c->b_hit_cap;
   const uint64_t waves = (uint64_t)persist_grid(c) * 4;
-  const size_t group_max = c->b_task_cap - (size_t)(2 * waves * kTailParts);  // room for the split tail
-  std::vector<Task> &grp = c->b_group;
-  for (size_t t = 0; t < tasks.size();) {
-    size_t k = 0;
-    double e = 0;
-    while (t + k < tasks.size() && k < group_max) {
-      const Task &T = tasks[t + k];
-      const double et = hpw[T.piece] * T.count * T.nw;
-      if (k && e + et > room) break;
-      e += et;
-      ++k;
+  const uint64_t group_max = c->b_task_cap - 2 * waves * kTailParts;  // room for the cut tail
+  uint64_t g0 = 0, gn = 0;  // the launch being formed: its first task and its count
+  double eh = 0;  // its expected raw hits
+  for (const TaskClass &k : cls) {
+    const PieceDesc &D = c->h_pieces[k.piece];
+    const Piece &P = pieces[b + k.piece];
+    const double hpw = D.nwin ? hits_per_prime(*par[P.unit], P.lo | 1) / D.nwin : 0.0;
+    const double et = hpw * k.count * k.nw;  // expected raw hits of one task of the class
+    uint64_t left = (uint64_t)k.nf * k.nseg;
+    while (left) {
+      uint64_t take = std::min(left, group_max - gn);
+      if (et > 0) {
+        const double fit = (room - eh) / et;  // tasks that still fit (may be huge)
+        if (fit < (double)take) take = fit > 0 ? (uint64_t)fit : 0;
+      }
+      if (take == 0 && gn == 0) take = 1;
+      if (take == 0) {  // the launch is full
+        int rc = run_tasks(c, g0, g0 + gn, raw, buf);
+        if (rc) return rc;
+        g0 += gn;
+        gn = 0;
+        eh = 0;
+        continue;
+      }
+      gn += take;
+      eh += et * (double)take;
+      left -= take;
     }
-    split_tail(tasks.data() + t, k, waves, grp);
-    int rc = run_tasks(c, grp.data(), (uint32_t)grp.size(), raw, buf);
-    if (rc) return rc;
-    t += k;
   }
-  return 0;
+  return run_tasks(c, g0, g0 + gn, raw, buf);
 }
 
 int ppsd_batch_prepare(ppsd_ctx *c, uint64_t pmax_all) {

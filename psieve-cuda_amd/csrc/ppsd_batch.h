@@ -20,9 +20,32 @@ struct PieceDesc {
 };
 
 // A unit of the work queue: count (<= 64*R) primes from pool offset
-// pieces[piece].prime_off + first, windows w0 .. w0+nw of that piece.
+// pieces[piece].prime_off + first, windows w0 .. w0+nw of that piece
+// (nw = 0: nothing to do).
 struct Task {
   uint32_t piece, first, count, w0, nw;
+};
+
+// A class of equal tasks of one piece: nf blocks of `count` primes at
+// first = f0 + i*fstep, times nseg segments of nw windows at w0 + j*nw;
+// task t_begin + i*nseg + j of the round's order is block i, segment j.
+struct TaskClass {
+  uint64_t t_begin;
+  uint32_t piece, f0, fstep, nf, count, w0, nw, nseg;
+};
+
+// The queue of one launch, built on the device from the round's classes
+// (ascending t_begin): tasks [t0, t1) of the round's order, except that the
+// last `tail` of them are each cut into up to kTailParts segments of at
+// least tail_min_windows windows and queued after the rest, one part of
+// every tail task in turn.  Writes the tasks and their number.
+struct ExpandArgs {
+  const TaskClass *classes;
+  uint32_t nclasses;
+  uint64_t t0, t1;
+  uint32_t tail, parts, tail_min_windows;
+  Task *out;
+  uint32_t *nout;
 };
 
 // Raw hit of the batched walk.
@@ -55,5 +78,7 @@ struct BatchArgs {
 hipError_t launch_persist(const BatchArgs &a, int r, int u, uint32_t grid, hipStream_t st);
 hipError_t launch_batch_filter(const BatchArgs &a, uint32_t grid, hipStream_t st);
 hipError_t launch_emit(const BHit *src, uint64_t n, const PieceDesc *pieces, BHit *dst, hipStream_t st);
+// Tasks of the launch described by a (ExpandArgs): (t1 - t0 - tail) + tail * parts of them.
+hipError_t launch_expand(const ExpandArgs &a, hipStream_t st);
 
 }  // namespace ppsd

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(256) void persist_walk_kernel(BatchArgs a) {
       return;
     }
     const Task T = a.tasks[t];
+    if (T.nw == 0) continue;  // an unused part of a cut tail task
     const PieceDesc P = a.pieces[T.piece];
     if (P.pre == PRE_RANGE) run_form<R, U, PRE_RANGE>(a, P, T, stage[wave], &fill[wave]);
     else if (P.pre == PRE_ODDLO) run_form<R, U, PRE_ODDLO>(a, P, T, stage[wave], &fill[wave]);
@@ -141,6 +142,51 @@ __global__ __launch_bounds__(256) void emit_kernel(const BHit *src, uint64_t n, 
     h.piece = pieces[h.piece].unit;
     dst[i] = h;
   }
+}
+
+// Task t of the round's order, from its class (binary search over t_begin).
+__device__ __forceinline__ Task task_of(const TaskClass *cl, uint32_t ncl, uint64_t t) {
+  uint32_t lo = 0, hi = ncl;  // cl[lo].t_begin <= t < cl[hi].t_begin (hi = ncl: past the end)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (cl[mid].t_begin <= t) lo = mid;
+    else hi = mid;
+  }
+  const TaskClass &C = cl[lo];
+  const uint64_t local = t - C.t_begin;
+  const uint32_t i = (uint32_t)(local / C.nseg), j = (uint32_t)(local - (uint64_t)i * C.nseg);
+  return Task{C.piece, C.f0 + i * C.fstep, C.count, C.w0 + j * C.nw, C.nw};
+}
+
+__global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a) {
+  const uint64_t n = a.t1 - a.t0, head = n - a.tail, total = head + (uint64_t)a.tail * a.parts;
+  const uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (q == 0) *a.nout = (uint32_t)total;
+  if (q >= total) return;
+  if (q < head) {
+    a.out[q] = task_of(a.classes, a.nclasses, a.t0 + q);
+    return;
+  }
+  // part `part` of tail task `i`: windows [nw*part/parts_i, nw*(part+1)/parts_i)
+  const uint64_t u = q - head;
+  const uint32_t i = (uint32_t)(u % a.tail), part = (uint32_t)(u / a.tail);
+  Task T = task_of(a.classes, a.nclasses, a.t0 + head + i);
+  const uint32_t parts = max(1u, min(a.parts, T.nw / a.tail_min_windows));
+  if (part >= parts) {
+    T.nw = 0;
+  } else {
+    const uint32_t w0 = (uint32_t)((uint64_t)T.nw * part / parts), w1 = (uint32_t)((uint64_t)T.nw * (part + 1) / parts);
+    T.w0 += w0;
+    T.nw = w1 - w0;
+  }
+  a.out[q] = T;
+}
+
+hipError_t launch_expand(const ExpandArgs &a, hipStream_t st) {
+  const uint64_t total = (a.t1 - a.t0 - a.tail) + (uint64_t)a.tail * a.parts;
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, (total + 255) / 256);
+  hipLaunchKernelGGL(expand_kernel, dim3(blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
 }
 
 template <int R, int U>

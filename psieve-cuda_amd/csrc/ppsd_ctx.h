@@ -134,8 +134,11 @@ struct ppsd_ctx {
   unsigned long long *h_sums[2] = {};
   uint32_t b_piece_cap = 0, b_task_cap = 0;
   PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
-  Task *b_tasks = nullptr, *h_tasks = nullptr;
-  uint32_t *b_ntasks = nullptr, *b_next = nullptr, *h_ntasks = nullptr;
+  Task *b_tasks = nullptr;                        // one launch's queue (launch_expand)
+  uint32_t *b_ntasks = nullptr, *b_next = nullptr;  // its length, its head
+  std::vector<ppsd::TaskClass> b_classes;          // a round's task classes (capacity kept)
+  ppsd::TaskClass *b_dclasses = nullptr, *h_classes = nullptr;  // on the device; pinned upload staging
+  uint32_t b_class_cap = 0;
   BHit *b_hits = nullptr, *b_kept = nullptr;
   uint64_t b_hit_cap = 0, b_hit_alloc = 0;
   unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
@@ -167,8 +170,6 @@ struct ppsd_ctx {
   uint64_t h_out_cap = 0;
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_emit = nullptr;
-  std::vector<Task> b_task_list;  // a round's tasks, built on the host (capacity kept)
-  std::vector<Task> b_group;      // one launch's tasks, its tail cut shorter (capacity kept)
   std::vector<uint64_t> b_unit_primes, b_unit_sums;
   uint32_t b_nunits = 0;
 };
