@@ -196,7 +196,7 @@ static int free_ctx(ppsd_ctx *c) {
   (void)hipFree(c->d_blockcount);
   (void)hipFree(c->d_blocksum);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < kPools; ++i) {
     if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
     if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
     (void)hipFree(c->b_pool[i]);
@@ -816,7 +816,7 @@ int ppsd::batch_alloc(ppsd_ctx *c) {
   c->b_piece_cap = 1u << 16;
   c->b_task_cap = 1u << 22;
   c->b_hit_cap = c->b_hit_alloc = (uint64_t)1 << 25;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < kPools; ++i) {
     HIP_TRY(hipMalloc(&c->b_pool[i], c->b_pool_cap * sizeof(uint64_t)));
     HIP_TRY(hipMalloc(&c->b_pool_count[i], sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&c->b_off[i], (c->b_piece_cap + 1) * sizeof(uint32_t)));
@@ -916,14 +916,18 @@ static BatchArgs batch_args(ppsd_ctx *c, int buf) {
   return a;
 }
 
-static uint32_t persist_grid(ppsd_ctx *c) {
+static uint32_t device_cus(ppsd_ctx *c) {
   if (!c->cus) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
       cus = 256;
     c->cus = (uint32_t)cus;
   }
-  return c->cus * (uint32_t)c->b_waves;  // 256-thread groups: b_waves waves per SIMD
+  return c->cus;
+}
+
+static uint32_t persist_grid(ppsd_ctx *c) {
+  return device_cus(c) * (uint32_t)c->b_waves;  // 256-thread groups: b_waves waves per SIMD
 }
 
 // The round's device sequence -- zero the counters and the queue head, drain
@@ -1174,6 +1178,7 @@ int ppsd::sieve_round(ppsd_ctx *c, const std::vector<Piece> &pieces, size_t b, s
     a.qinv = c->d_spinv;
     a.bitmap = c->b_rbitmap;
     a.block = 256;  // one wave per SIMD: fits beside the persistent walk's waves
+    a.grid = 2 * device_cus(c);  // two workgroups a CU, as many as the slots the walk leaves free at its default shape
     HIP_TRY(launch_round_sieve(a, c->b_rblockcount, c->b_rblocksum, c->b_pool[buf], (uint32_t)c->b_pool_cap,
                                c->b_pool_count[buf], c->b_off[buf], c->b_sums[buf], st));
   }
@@ -1277,7 +1282,7 @@ int ppsd_batch_prepare(ppsd_ctx *c, uint64_t pmax_all) {
   if (rc) return rc;
   // both pools' round graphs (captured again only when the kernel shape or
   // the ring changed), not at the first round that uses each
-  for (int buf = 0; buf < 2; ++buf) {
+  for (int buf = 0; buf < kPools; ++buf) {
     rc = round_graph(c, buf);
     if (rc) return rc;
   }
@@ -1335,30 +1340,30 @@ int ppsd_batch_run(ppsd_ctx *c, const ppsd_params *units, uint32_t nunits, int r
     rounds.push_back({b, e});
     b = e;
   }
-  // Round i+1 is sieved on stream2 into the other pool while round i walks
-  // on the context's stream; round i+1's sieve is enqueued only after round
-  // i-1's walk (which read that pool) has finished.
+  // Round i is walked on the context's stream from pool i mod kPools while
+  // rounds i+1 and i+2 are sieved on stream2 into the others; round i+2's
+  // sieve is enqueued only after round i-1's walk (which read that pool)
+  // has finished.
   auto sieve = [&](size_t i) {
-    const int buf = (int)(i & 1);
+    const int buf = (int)(i % kPools);
     return sieve_round(c, pieces, rounds[i].first, rounds[i].second, buf, c->stream2, c->ev2[buf][0],
                        c->ev2[buf][1]);
   };
-  if (!rounds.empty()) {
-    rc = sieve(0);
+  for (size_t i = 0; i < std::min<size_t>(kPools - 1, rounds.size()); ++i) {
+    rc = sieve(i);
     if (rc) return rc;
   }
   for (size_t i = 0; i < rounds.size(); ++i) {
-    const int buf = (int)(i & 1);
+    const int buf = (int)(i % kPools);
     const auto [b, e] = rounds[i];
-    HIP_TRY(hipEventSynchronize(c->ev2[buf][1]));
-    if (i + 1 < rounds.size()) {
-      rc = sieve(i + 1);
+    if (i + kPools - 1 < rounds.size()) {
+      rc = sieve(i + kPools - 1);
       if (rc) return rc;
     }
     rc = walk_round(c, pieces, b, e, par, raw != 0, buf, c->ev2[buf][0], c->ev2[buf][1]);
     if (rc == 1) {  // pool overflow (rare): this round's pieces again, in two halves, on pool `buf`
       if (e - b == 1) return set_err(PPSD_ERR_FAILURE, "one piece overflowed the prime pool");
-      HIP_TRY(hipStreamSynchronize(c->stream2));  // the sieve buffers are shared with round i+1's sieve
+      HIP_TRY(hipStreamSynchronize(c->stream2));  // the sieve buffers are shared with the next rounds' sieves
       const size_t mid = b + (e - b) / 2;
       for (const auto &[x, y] : {std::make_pair(b, mid), std::make_pair(mid, e)}) {
         rc = sieve_round(c, pieces, x, y, buf, c->stream, c->ev[2], c->ev[3]);

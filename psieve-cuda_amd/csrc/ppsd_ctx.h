@@ -29,6 +29,11 @@ std::vector<uint16_t> pow2_table(const std::vector<SmallPrime> &sm);
 
 enum class Pending { none, primes, range };
 
+// Prime pools of the batched walk: round i is walked from pool i mod kPools
+// while rounds i+1 and i+2 are sieved into the others, so that a round's
+// sieve has a whole walk to finish in before its own walk needs it.
+constexpr int kPools = 3;
+
 // An allocator whose construct() without arguments default-initialises, so
 // that resize() before a bulk copy does not zero the new elements first.
 template <class T>
@@ -124,14 +129,14 @@ struct ppsd_ctx {
   bool b_ready = false;
   uint32_t cus = 0;  // compute units of the device (read on first use)
   uint64_t b_pool_cap = 0;
-  // two prime pools: the next round is sieved (on stream2) while the
+  // kPools prime pools: the next rounds are sieved (on stream2) while the
   // current one is walked (on stream)
-  uint64_t *b_pool[2] = {};
-  uint32_t *b_pool_count[2] = {};
+  uint64_t *b_pool[ppsd::kPools] = {};
+  uint32_t *b_pool_count[ppsd::kPools] = {};
   hipStream_t stream2 = nullptr;
-  hipEvent_t ev2[2][2] = {};                  // per pool: sieve begin/end on stream2
-  uint32_t *h_off[2] = {};                    // per pool, pinned: copies of b_off/b_sums
-  unsigned long long *h_sums[2] = {};
+  hipEvent_t ev2[ppsd::kPools][2] = {};  // per pool: sieve begin/end on stream2
+  uint32_t *h_off[ppsd::kPools] = {};    // per pool, pinned: copies of b_off/b_sums
+  unsigned long long *h_sums[ppsd::kPools] = {};
   uint32_t b_piece_cap = 0, b_task_cap = 0;
   PieceDesc *b_pieces = nullptr, *h_pieces = nullptr;
   Task *b_tasks = nullptr;                        // one launch's queue (launch_expand)
@@ -142,19 +147,19 @@ struct ppsd_ctx {
   BHit *b_hits = nullptr, *b_kept = nullptr;
   uint64_t b_hit_cap = 0, b_hit_alloc = 0;
   unsigned long long *b_counts = nullptr;  // [0] raw, [1] kept
-  uint32_t *b_off[2] = {};              // per pool: prime offset of each piece (and the count after the last)
-  unsigned long long *b_sums[2] = {};   // per pool: prime sum of each piece
+  uint32_t *b_off[ppsd::kPools] = {};              // per pool: prime offset of each piece (and the count after the last)
+  unsigned long long *b_sums[ppsd::kPools] = {};   // per pool: prime sum of each piece
   // a round's sieve (sieve_round): one bitmap of up to kRoundSegs segments
   // per launch, and per pool the pieces and the segment -> piece table
   uint32_t *b_rbitmap = nullptr, *b_rblockcount = nullptr;
   unsigned long long *b_rblocksum = nullptr;
-  ppsd::RoundPiece *d_rpieces[2] = {}, *h_rpieces[2] = {};
-  uint32_t *d_rsegs[2] = {}, *h_rsegs[2] = {};
-  uint32_t *d_rorder[2] = {}, *h_rorder[2] = {};  // per launch, its pieces by descending nq
+  ppsd::RoundPiece *d_rpieces[ppsd::kPools] = {}, *h_rpieces[ppsd::kPools] = {};
+  uint32_t *d_rsegs[ppsd::kPools] = {}, *h_rsegs[ppsd::kPools] = {};
+  uint32_t *d_rorder[ppsd::kPools] = {}, *h_rorder[ppsd::kPools] = {};  // per launch, its pieces by descending nq
   uint32_t b_rseg_cap = 0;
-  hipGraph_t b_graph[2] = {};           // per pool: the round graph
-  hipGraphExec_t b_exec[2] = {};
-  int b_graph_key[2] = {-1, -1};
+  hipGraph_t b_graph[ppsd::kPools] = {};           // per pool: the round graph
+  hipGraphExec_t b_exec[ppsd::kPools] = {};
+  int b_graph_key[ppsd::kPools] = {-1, -1, -1};
   int b_r = 1, b_u = 16, b_waves = 6;  // defaults: the tuner's choice on MI355X
   uint32_t b_seg_min = 2048, b_seg_max = 65536;
   BHit *b_out = nullptr;  // the batch's results so far, on the device, unit-indexed

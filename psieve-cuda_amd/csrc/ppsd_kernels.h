@@ -104,6 +104,7 @@ struct RoundSieveArgs {
   const uint64_t *qinv;       // floor((2^64-1)/q) of the LDS-sieved ones
   uint32_t *bitmap;           // nseg * kSegWords words; set bit = composite
   uint32_t block;             // threads per LDS-sieve workgroup: 256 or 1024
+  uint32_t grid;              // at most this many workgroups per sieve launch (0: no cap)
 };
 
 hipError_t launch_walk(const WalkArgs &a, int sign, int r, int u, uint32_t grid, uint32_t block,

@@ -228,12 +228,14 @@ __global__ __launch_bounds__(BS) void sieve_small_kernel(SieveArgs a) {
 template <int BS>
 __global__ __launch_bounds__(BS) void round_sieve_small_kernel(RoundSieveArgs a) {
   __shared__ uint32_t seg[kSegWords];
-  const uint32_t g = blockIdx.x;
-  const RoundPiece P = a.pieces[a.seg_piece[g]];
-  const uint64_t b0 = (uint64_t)g * kSegBits - P.bit0;  // within the piece
-  const uint64_t rem = P.nbits - b0;
-  const uint32_t nb = rem < kSegBits ? (uint32_t)rem : kSegBits;
-  sieve_segment(seg, P.lo + 2 * b0, nb, a.q, a.qinv, P.nq_small, a.bitmap + (uint64_t)g * kSegWords, kSegWords);
+  for (uint32_t g = blockIdx.x; g < a.nseg; g += gridDim.x) {  // workgroup-uniform
+    const RoundPiece P = a.pieces[a.seg_piece[g]];
+    const uint64_t b0 = (uint64_t)g * kSegBits - P.bit0;  // within the piece
+    const uint64_t rem = P.nbits - b0;
+    const uint32_t nb = rem < kSegBits ? (uint32_t)rem : kSegBits;
+    sieve_segment(seg, P.lo + 2 * b0, nb, a.q, a.qinv, P.nq_small, a.bitmap + (uint64_t)g * kSegWords, kSegWords);
+    __syncthreads();  // seg is read out before the next segment clears it
+  }
 }
 
 // The large primes of a round launch, one per thread: q = q[t] for t in
@@ -242,18 +244,19 @@ __global__ __launch_bounds__(BS) void round_sieve_small_kernel(RoundSieveArgs a)
 // is formed once per prime, so that each piece's first multiple costs a
 // multiply-high and one correction instead of a 64-bit remainder.
 __global__ __launch_bounds__(256) void round_sieve_large_kernel(RoundSieveArgs a) {
-  const uint64_t t = (uint64_t)a.qlo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (t >= a.qhi) return;
-  const uint32_t q = a.q[t];
-  const uint64_t inv = ~0ull / q;
-  for (uint32_t j = a.p0; j < a.p1; ++j) {
-    const RoundPiece P = a.pieces[a.by_nq[j]];
-    if (P.nq <= t) break;
-    if (t < P.nq_small) continue;  // marked in LDS for this piece
-    const uint64_t m = first_multiple_inv(P.lo, q, inv);
-    for (uint64_t b = (m - P.lo) >> 1; b < P.nbits; b += q) {
-      const uint64_t bit = P.bit0 + b;
-      atomicOr(&a.bitmap[bit >> 5], 1u << (bit & 31));
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)a.qlo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < a.qhi; t += stride) {
+    const uint32_t q = a.q[t];
+    const uint64_t inv = ~0ull / q;
+    for (uint32_t j = a.p0; j < a.p1; ++j) {
+      const RoundPiece P = a.pieces[a.by_nq[j]];
+      if (P.nq <= t) break;
+      if (t < P.nq_small) continue;  // marked in LDS for this piece
+      const uint64_t m = first_multiple_inv(P.lo, q, inv);
+      for (uint64_t b = (m - P.lo) >> 1; b < P.nbits; b += q) {
+        const uint64_t bit = P.bit0 + b;
+        atomicOr(&a.bitmap[bit >> 5], 1u << (bit & 31));
+      }
     }
   }
 }
@@ -460,12 +463,19 @@ hipError_t launch_round_sieve(const RoundSieveArgs &a, uint32_t *blockcount, uns
                               unsigned long long *psum, hipStream_t st) {
   const uint32_t nblocks = a.nseg * kBlocksPerSeg;
   if (a.nseg) {
+    // a.grid caps both sieve launches (one workgroup per segment or per 256
+    // large primes otherwise): beside a persistent walk, a launch of
+    // thousands of workgroups would take every slot the walk's end frees and
+    // hold off the next walk until it drained
+    const uint32_t gs = a.grid ? std::min(a.nseg, a.grid) : a.nseg;
     if (a.block == 256)
-      hipLaunchKernelGGL(round_sieve_small_kernel<256>, dim3(a.nseg), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(round_sieve_small_kernel<256>, dim3(gs), dim3(256), 0, st, a);
     else
-      hipLaunchKernelGGL(round_sieve_small_kernel<1024>, dim3(a.nseg), dim3(1024), 0, st, a);
-    if (a.qhi > a.qlo)
-      hipLaunchKernelGGL(round_sieve_large_kernel, dim3((a.qhi - a.qlo + 255) / 256), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(round_sieve_small_kernel<1024>, dim3(gs), dim3(1024), 0, st, a);
+    if (a.qhi > a.qlo) {
+      const uint32_t gl = (a.qhi - a.qlo + 255) / 256;
+      hipLaunchKernelGGL(round_sieve_large_kernel, dim3(a.grid ? std::min(gl, a.grid) : gl), dim3(256), 0, st, a);
+    }
     hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(256), 0, st, a.bitmap, nblocks * kScanBlock, blockcount);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, st, blockcount, nblocks, count_d);
     hipLaunchKernelGGL(round_compact_kernel, dim3(nblocks), dim3(256), 0, st, a, (const uint32_t *)blockcount, out,
