@@ -26,9 +26,9 @@ import ppsd_units  # noqa: E402
 CACHE_DIR = os.path.join(os.path.dirname(HERE), "tuning")
 
 PLAIN_GRID = dict(r=(1, 2), u=(4, 8, 16), wg=(64, 128, 256))
-# waves per SIMD below the 8 a SIMD holds leave room for the next round's
-# sieve, which runs beside the walk on a second stream
-BATCH_GRID = dict(r=(1, 2), u=(4, 8, 16), waves=(6, 7, 8), seg=((2048, 65536),))
+# waves per SIMD below the 8 a SIMD holds leave room for the next rounds'
+# sieves, which run beside the walk on a second stream
+BATCH_GRID = dict(r=(1, 2), u=(4, 8, 16), waves=(5, 6, 7, 8), seg=((2048, 65536),))
 
 # the plain sample: one batch of the 2^50 workload (k <= 1e4, n <= 6e6)
 PLAIN_SAMPLE = dict(pmin=2**50, pmax=2**50 + 5 * 10**10, kmin=1201, kmax=9999, nmin=50000, nmax=6000000)
