@@ -274,6 +274,23 @@ __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R
   constexpr int RPRE = PRE == PRE_RANGE ? PRE_RANGE : PRE_EXACT;
   const uint32_t s = a.s;
   uint32_t w = 0;
+  if (!KEEP) {
+    // Two chunks a turn: each chunk's snapshot must outlive its steps, so
+    // one chunk at a time ended with a copy of the state into the next
+    // snapshot's registers (a v_mov_b64 per chunk); with two, the state
+    // alternates between two register pairs and no copy is left.
+    for (; w + 2 * U <= nwin; w += 2 * U) {
+      LaneStates<R> hist[U];
+      const LaneStates<R> snap0 = S;
+      if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist))
+        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap0, n, U, a, sink);
+      n += U * s;
+      const LaneStates<R> snap1 = S;
+      if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist))
+        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap1, n, U, a, sink);
+      n += U * s;
+    }
+  }
   for (; w + U <= nwin; w += U) {
     LaneStates<R> hist[U];
     const LaneStates<R> snap = S;
