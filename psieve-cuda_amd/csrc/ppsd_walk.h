@@ -274,33 +274,34 @@ __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R
   constexpr int RPRE = PRE == PRE_RANGE ? PRE_RANGE : PRE_EXACT;
   const uint32_t s = a.s;
   uint32_t w = 0;
-  if (!KEEP) {
-    // Two chunks a turn: each chunk's snapshot must outlive its steps, so
-    // one chunk at a time ended with a copy of the state into the next
-    // snapshot's registers (a v_mov_b64 per chunk); with two, the state
-    // alternates between two register pairs and no copy is left.
-    for (; w + 2 * U <= nwin; w += 2 * U) {
-      LaneStates<R> hist[U];
-      const LaneStates<R> snap0 = S;
-      if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist))
-        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap0, n, U, a, sink);
-      n += U * s;
-      const LaneStates<R> snap1 = S;
-      if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist))
-        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap1, n, U, a, sink);
-      n += U * s;
-    }
-  }
-  for (; w + U <= nwin; w += U) {
+  // One chunk: its pre-test while stepping, then, if it fired, its windows
+  // checked one by one -- from the states kept in hist (KEEP) or by walking
+  // the chunk again from its snapshot.
+  auto chunk = [&](uint32_t n0) {
     LaneStates<R> hist[U];
     const LaneStates<R> snap = S;
     if (test_and_step<SIGN, R, U, S32, PRE, FUSED>(L, S, a, hist)) {
       if (KEEP)
-        recheck<SIGN, R, U, RPRE>(L, hist, n, a, sink);
+        recheck<SIGN, R, U, RPRE>(L, hist, n0, a, sink);
       else
-        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n, U, a, sink);
+        rescan<SIGN, R, S32, FUSED, RPRE>(L, snap, n0, U, a, sink);
     }
+  };
+  // Two chunks a turn: a chunk's first state must stay in its registers
+  // through the chunk (it is the snapshot, or the first kept state), so with
+  // one chunk a turn the loop ended by copying the last state into them (a
+  // v_mov_b64 per chunk, which takes an issue slot); with two, the state
+  // alternates between two register pairs and no copy is left.
+  for (; w + 2 * U <= nwin; w += 2 * U) {
+    chunk(n);
     n += U * s;
+    chunk(n);
+    n += U * s;
+  }
+  if (w + U <= nwin) {
+    chunk(n);
+    n += U * s;
+    w += U;
   }
   for (; w < nwin; ++w, n += s) {
     LaneStates<R> hist[1];
