@@ -184,9 +184,10 @@ const char *ppsd_last_error(void);
  * ppsd_batch_run searches every unit completely: each is a full ppsd_params
  * (its own p-, k- and n-range, sign, window, k filter) accepted by ppsd_open,
  * with no partial sieve (qmax 0 or >= isqrt(pmax-1)).  The units' primes are
- * sieved on the device into one pool; one persistent kernel then drains a
- * device queue of (unit, 64*r primes, n-segment) tasks, so that the whole
- * GPU works on whatever mix of units is present.  Results are those of
+ * sieved on the device a round at a time into prime pools (the next rounds
+ * beside the current one's walk); one persistent kernel then drains a device
+ * queue of (unit, 64*r primes, n-segment) tasks, so that the whole GPU works
+ * on whatever mix of units is present.  Results are those of
  * searching each unit alone with ppsd_submit_range: the same factors (or,
  * raw != 0, the same divisors before test_factor) -- the order differs.
  * Synchronous; the context must have no outstanding submission and must not
@@ -194,8 +195,9 @@ const char *ppsd_last_error(void);
 int ppsd_batch_run(ppsd_ctx *ctx, const ppsd_params *units, uint32_t nunits, int raw);
 
 /* The batched path's one-time set-up, done ahead of the first ppsd_batch_run
- * that would otherwise do it: its device buffers (about 6 GiB) and pinned
- * staging, the sieving primes for every unit with pmax <= pmax_all, and the
+ * that would otherwise do it: its device buffers (about 6.4 GB) and pinned
+ * host buffers (about 560 MB, most of it the mirror the results are copied
+ * to during the run), the sieving primes for every unit with pmax <= pmax_all, and the
  * round graphs of the current kernel shape and ring capacity.  Optional:
  * ppsd_batch_run does whatever is missing.  No counterpart in the reference,
  * whose host allocates per device in cuda_app_init (pps/appcu.cu:213-404). */
