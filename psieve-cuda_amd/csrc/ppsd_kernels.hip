@@ -222,12 +222,23 @@ __global__ __launch_bounds__(BS) void sieve_small_kernel(SieveArgs a) {
   sieve_segment(seg, a.lo + 2 * b0, nb, a.q, a.qinv, a.nq_small, a.bitmap + b0 / 32, (nb + 31) / 32);
 }
 
+// The kernels of a round sieve run on a second stream beside the batched
+// walk, whose waves keep every SIMD's VALU issue busy.  A SIMD issues from
+// the highest-priority ready wave, the oldest among equals: at the default
+// priority a sieve wave next to the walk's older waves issues almost never,
+// and the sieve finishes only after the walk has, in the gap before the next
+// walk.  At the highest priority it issues whenever it is ready; it spends
+// most of its time waiting on LDS and memory atomics, so the instructions it
+// takes from the walk are few.
+__device__ __forceinline__ void beside_walk_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // One workgroup per segment of a round launch: the segment's piece from
 // seg_piece; all kSegWords words are written (a piece's last segment is
 // padded with composites), so the compaction can run over whole blocks.
 template <int BS>
 __global__ __launch_bounds__(BS) void round_sieve_small_kernel(RoundSieveArgs a) {
   __shared__ uint32_t seg[kSegWords];
+  beside_walk_priority();
   for (uint32_t g = blockIdx.x; g < a.nseg; g += gridDim.x) {  // workgroup-uniform
     const RoundPiece P = a.pieces[a.seg_piece[g]];
     const uint64_t b0 = (uint64_t)g * kSegBits - P.bit0;  // within the piece
@@ -244,6 +255,7 @@ __global__ __launch_bounds__(BS) void round_sieve_small_kernel(RoundSieveArgs a)
 // is formed once per prime, so that each piece's first multiple costs a
 // multiply-high and one correction instead of a 64-bit remainder.
 __global__ __launch_bounds__(256) void round_sieve_large_kernel(RoundSieveArgs a) {
+  beside_walk_priority();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)a.qlo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < a.qhi; t += stride) {
     const uint32_t q = a.q[t];
@@ -316,7 +328,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &t
   return before + x - v;
 }
 
+// count_kernel and scan_kernel also end every round sieve, so they take the
+// round sieve's wave priority (the plain path's sieve uses them too).
 __global__ __launch_bounds__(256) void count_kernel(const uint32_t *bitmap, uint32_t nwords, uint32_t *blockcount) {
+  beside_walk_priority();
   const uint32_t idx = blockIdx.x * kScanBlock + threadIdx.x;
   const uint32_t c = idx < nwords ? (uint32_t)__popc(~bitmap[idx]) : 0u;
   uint32_t total;
@@ -327,6 +342,7 @@ __global__ __launch_bounds__(256) void count_kernel(const uint32_t *bitmap, uint
 // Exclusive scan of blockcount in place, offset by the running count in
 // *count_d, which is then advanced by the chunk's total.  One workgroup.
 __global__ __launch_bounds__(256) void scan_kernel(uint32_t *blockcount, uint32_t nblocks, uint32_t *count_d) {
+  beside_walk_priority();
   const uint32_t per = (nblocks + 255) / 256;
   const uint32_t beg = threadIdx.x * per;
   const uint32_t end = beg + per < nblocks ? beg + per : nblocks;
@@ -417,6 +433,7 @@ hipError_t launch_compact32(const uint32_t *bitmap, uint64_t lo, uint64_t nbits,
 // to the piece of its segment.
 __global__ __launch_bounds__(256) void round_compact_kernel(RoundSieveArgs a, const uint32_t *blockoff, uint64_t *out,
                                                             uint32_t cap, unsigned long long *blocksum) {
+  beside_walk_priority();
   const uint32_t blk = blockIdx.x;
   const RoundPiece P = a.pieces[a.seg_piece[blk / kBlocksPerSeg]];
   const uint64_t idx = (uint64_t)blk * kScanBlock + threadIdx.x;
@@ -445,6 +462,7 @@ __global__ __launch_bounds__(64) void round_piece_totals_kernel(RoundSieveArgs a
                                                                 const unsigned long long *blocksum, uint32_t nblocks,
                                                                 const uint32_t *count_d, uint32_t *off,
                                                                 unsigned long long *psum) {
+  beside_walk_priority();
   const uint32_t i = a.p0 + blockIdx.x;
   const RoundPiece P = a.pieces[i];
   const uint32_t first = (uint32_t)(P.bit0 / (kScanBlock * 32));
