@@ -64,9 +64,11 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const PieceDesc &P,
   const WindowRule rule{P.kmin, P.kmax, P.nmax, P.s, P.kmax << (P.s - 1)};
   start_states<SIGN, R>(L, S, n0, P.s, rule.bound);
   const StagedSink<PieceSink, kBatchStage> sink{PieceSink{a.hits, a.hit_count, a.hit_cap, T.piece}, stage, fill};
-  // states not kept: the persistent kernel holds every form, and its
-  // registers (so its waves per SIMD) are set by the largest
-  walk_windows<SIGN, R, U, S32, PRE, false>(L, S, n0, T.nw, rule, sink);
+  // A chunk's states are kept for its re-check at U = 4 (about 3 of the 7
+  // instructions of a re-walked window saved; at r = 1 68 VGPRs, still 7
+  // waves per SIMD) and re-walked from the snapshot at U = 8 and 16, where U kept
+  // states would cost the kernel, which holds every form, resident waves.
+  walk_windows<SIGN, R, U, S32, PRE, (U <= 4)>(L, S, n0, T.nw, rule, sink);
 }
 
 template <int R, int U, int PRE>

@@ -267,7 +267,8 @@ __device__ __forceinline__ bool test_and_step(const LanePrimes<R> &L, LaneStates
 // A chunk whose pre-test fired is checked window by window: with KEEP from
 // the states it passed through (U*R more state registers, no re-stepping),
 // else by re-walking it from a snapshot.  KEEP pays where false alarms are
-// frequent (the range form, on up to a tenth of the chunks).
+// frequent (the range form, on up to a tenth of the chunks) and where the
+// chunk is short enough that its U states cost no resident waves.
 template <int SIGN, int R, int U, bool S32, int PRE, bool FUSED, bool KEEP, class Sink>
 __device__ __forceinline__ void walk_chunks(const LanePrimes<R> &L, LaneStates<R> &S, uint32_t n, uint32_t nwin,
                                             const WindowRule &a, const Sink &sink) {
